@@ -1,0 +1,312 @@
+"""Benchmark of the FlashKNN grid-kNN path on one MI355X.
+
+Headline (value): exact kNN query throughput, M queries/s, on a synthetic
+S3DIS-like room of ~100k fp32 points, queries = all points, K = 16, idx
+bit-exact against brute force (the first pass scans the 27-cell block, the
+queries it cannot certify are finished by shell expansion; both kernels are
+inside the timed step).  Inputs and the built grid are resident in HBM; the
+grid construction is timed separately (``extra.build_ms``).
+
+One step = one full query of all points.  W warmup steps, then K timed
+steps bracketed by torch.cuda.synchronize() (one GPU, no barrier needed).
+
+Also reported in ``extra``: K=32, the LiDAR sweep, the uniform 8192 case,
+the reference's 27-cell FlashKNN.query semantics, the 16-cloud batched
+workload (graph replay vs 16 per-cloud launches), and the differentiable
+hierarchy fwd+bwd.  ``roofline`` is for the dominant kernel (the first
+pass, k_query<16>), timed alone with HIP events on its stream.
+``cpu_baseline`` times the reference's CPU brute-force path (torch.cdist +
+topk in query blocks of 10,000, as FlashKNN/functions/utils.py:5-82) on a
+bounded query sample on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("M queries/s at 100k pts K=16 (bit-exact idx); batched vs per-cloud; "
+          "fwd+bwd ms")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def events_ms(fn, reps, stream=None):
+    """Median ms of fn() measured with HIP events on the current stream."""
+    import torch
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * reps)]
+    for i in range(reps):
+        ev[2 * i].record()
+        fn()
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    t = sorted(ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(reps))
+    return t[reps // 2], sum(t) / reps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; the kNN path "
+                          "shards by cloud (see DESIGN.md) but no multi-GPU figure is "
+                          "produced", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import flashknn_exprepo_amd as fk
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd import synthetic
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    fk.load()
+    T = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+
+    # ---------------------------------------------------------- headline
+    log("generating room (~100k points)")
+    room = synthetic.room(seed=48)
+    n = len(room)
+    xyz = T(room)
+    off = T(np.array([n], np.int32))
+    K = 16
+    eng = fk.ExactKNN()
+    cell = fk.cell_sizes(xyz, off, K)
+    inv = (1.0 / cell.double()).float()
+    g = eng.grid
+    log("build")
+    build = lambda: g.build_xyz(xyz, off, inv)  # noqa: E731
+    build()
+    build_ms, _ = events_ms(build, 10)
+    out_i = torch.empty((n, K), dtype=torch.int64, device=dev)
+    out_d = torch.empty((n, K), dtype=torch.float32, device=dev)
+
+    def step():
+        g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
+                out_dist=out_d)
+
+    log(f"headline: n={n} K={K} warmup={args.warmup} steps={args.steps}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = n / (ms_per_step * 1e-3) / 1e6
+    stats = g.query_stats()
+
+    # bit-exactness of the measured output on a sample (oracle as checker)
+    from oracle import oracle as orc
+    sample = np.random.default_rng(0).choice(n, 2000, replace=False)
+    ri, rd = orc.brute_knn(room, [n], room[sample], [2000], K)
+    exact_ok = bool(np.array_equal(out_i.cpu().numpy()[sample], ri))
+
+    # dominant kernel alone (first pass), HIP events on the current stream
+    def pass1():
+        g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
+                out_dist=out_d, passes=2)
+
+    def reset():
+        g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
+                out_dist=out_d, passes=1)
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    for i in range(args.steps):
+        reset()
+        ev[2 * i].record()
+        pass1()
+        ev[2 * i + 1].record()
+    torch.cuda.synchronize()
+    p1 = [ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)]
+    p1_ms = sum(p1) / len(p1)
+    ncells = g.info()["cells"]
+    bytes_q = 12 * n + 12 * n + 8 * K * n + 4 * (ncells + 1)
+    achieved = bytes_q / (p1_ms * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "query_room100k_k16_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": traffic, "kernel": "k_query<16> (first pass)",
+                "kernel_ms": round(p1_ms, 5), "algorithmic_bytes": bytes_q}
+
+    extra = {"build_ms": round(build_ms, 4), "cells": ncells,
+             "certified_first_pass": stats["certified_first_pass"],
+             "expanded": stats["expanded"], "bitexact_sample_2000": exact_ok,
+             "cell_side_m": round(float(cell[0]), 5)}
+
+    if not args.quick:
+        def qps(grid, k, mode, reps=20, **kw):
+            oi = torch.empty((grid.num_points, k), dtype=torch.int64, device=dev)
+            od = torch.empty((grid.num_points, k), dtype=torch.float32, device=dev)
+            f = lambda: grid.query(k, mode=mode, out_idx=oi, out_dist=od, **kw)  # noqa: E731
+            f()
+            med, _ = events_ms(f, reps)
+            return grid.num_points / (med * 1e-3) / 1e6, med
+
+        # K = 32 on the same room
+        e32 = fk.ExactKNN()
+        e32.grid.build_xyz(xyz, off, (1.0 / fk.cell_sizes(xyz, off, 32).double()).float())
+        v, ms = qps(e32.grid, 32, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
+        extra["room_k32_Mqps"], extra["room_k32_ms"] = round(v, 2), round(ms, 4)
+
+        # reference semantics: FlashKNN.query, voxel 4 cm, num_down 2 (27 cells)
+        grid_c = torch.floor(xyz / 0.04).long()
+        gc = fk.Grid()
+        gc.build_cells(grid_c, None, xyz, num_down=2)
+        v, ms = qps(gc, K, F.FK_MODE_CELLS)
+        extra["room_27cell_alpha4_Mqps"], extra["room_27cell_alpha4_ms"] = round(v, 2), round(ms, 4)
+        extra["room_27cell_mean_candidates"] = round(gc.candidate_count(1) / n, 2)
+
+        # LiDAR sweep
+        log("lidar")
+        lid = synthetic.lidar(seed=49)
+        lx = T(lid)
+        lo = T(np.array([len(lid)], np.int32))
+        el = fk.ExactKNN()
+        el.grid.build_xyz(lx, lo, (1.0 / fk.cell_sizes(lx, lo, K).double()).float())
+        v, ms = qps(el.grid, K, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
+        extra["lidar_n"] = len(lid)
+        extra["lidar_k16_Mqps"], extra["lidar_k16_ms"] = round(v, 2), round(ms, 4)
+
+        # uniform 8192
+        uni = synthetic.uniform(8192, 47)
+        ux, uo = T(uni), T(np.array([8192], np.int32))
+        eu = fk.ExactKNN()
+        eu.grid.build_xyz(ux, uo, (1.0 / fk.cell_sizes(ux, uo, K).double()).float())
+        v, ms = qps(eu.grid, K, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
+        extra["uniform8192_k16_Mqps"], extra["uniform8192_k16_ms"] = round(v, 2), round(ms, 4)
+
+        # batched 16 uneven clouds: graph replay vs per-cloud launches
+        log("batched")
+        clouds, ks = synthetic.batch(seed=50)
+        bx, bo = synthetic.pack(clouds)
+        bk = fk.BatchedKNN(T(bx), T(bo), ks, exact=True)
+        bk.run()
+        bk.capture()
+        rep_med, _ = events_ms(bk.replay, 20)
+        b_idx = bk.out_idx.clone()
+        engines = []
+        start = 0
+        for c, (cl, k) in enumerate(zip(clouds, ks)):
+            e = fk.ExactKNN()
+            cx = T(cl)
+            co = T(np.array([len(cl)], np.int32))
+            e.grid.build_xyz(cx, co, (1.0 / bk.cell[c:c + 1].double()).float())
+            oi = torch.empty((len(cl), k), dtype=torch.int64, device=dev)
+            od = torch.empty((len(cl), k), dtype=torch.float32, device=dev)
+            engines.append((e.grid, k, oi, od, start))
+            start += len(cl)
+
+        def per_cloud():
+            for gr, k, oi, od, _ in engines:
+                gr.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_SELF, out_idx=oi,
+                         out_dist=od)
+
+        pc_med, _ = events_ms(per_cloud, 20)
+        same = True
+        for c, (gr, k, oi, od, st) in enumerate(engines):
+            bi, _ = bk.per_cloud(c)
+            same &= bool(torch.equal(bi - st, oi))
+        nb = len(bx)
+        extra.update({"batched_points": nb, "batched_graph_ms": round(rep_med, 4),
+                      "batched_Mqps": round(nb / (rep_med * 1e-3) / 1e6, 2),
+                      "per_cloud_ms": round(pc_med, 4),
+                      "per_cloud_Mqps": round(nb / (pc_med * 1e-3) / 1e6, 2),
+                      "batched_speedup_vs_per_cloud": round(pc_med / rep_med, 3),
+                      "batched_idx_bitexact_vs_per_cloud": same,
+                      "batched_k": ks})
+        del b_idx
+
+        # differentiable hierarchy fwd+bwd
+        log("hierarchy")
+        stages = [T(s) for s in synthetic.hierarchy(seed=51)]
+        feats = [torch.randn(s.shape[0], 64, device=dev) for s in stages]
+        wts = [torch.randn(s.shape[0], 64, device=dev) for s in stages]
+
+        def fwd_bwd():
+            loss = 0.0
+            leaves = []
+            for l in range(len(stages) - 1):
+                q = stages[l].clone().requires_grad_(True)
+                p = stages[l + 1].clone().requires_grad_(True)
+                f = feats[l + 1].clone().requires_grad_(True)
+                leaves += [q, p, f]
+                idx, d = fk.knn(p.detach(), torch.tensor([p.shape[0]], dtype=torch.int32,
+                                                         device=dev), 16,
+                                q.detach(), torch.tensor([q.shape[0]], dtype=torch.int32,
+                                                         device=dev))
+                out = fk.idw_interpolate(f, q, p, idx, eps=1e-8, d=d)
+                loss = loss + (out * wts[l]).sum()
+            loss.backward()
+            return leaves
+
+        fwd_bwd()
+        h_med, _ = events_ms(fwd_bwd, 10)
+        extra["hierarchy_sizes"] = [int(s.shape[0]) for s in stages]
+        extra["hierarchy_fwd_bwd_ms"] = round(h_med, 4)
+
+    # ------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline")
+        cores = os.cpu_count()
+        torch.set_num_threads(cores)
+        s = torch.from_numpy(room)
+        qn = 8192
+        qs = torch.from_numpy(room[np.random.default_rng(1).choice(n, qn, replace=False)])
+
+        def cpu_run():
+            for b0 in range(0, qn, 10000):
+                d = torch.cdist(qs[b0:b0 + 10000], s)
+                torch.topk(d, K, dim=1, largest=False)
+
+        cpu_run()
+        reps = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            cpu_run()
+            reps.append(time.perf_counter() - t0)
+        tc = sorted(reps)[1]
+        cpu = {"value": round(qn / tc / 1e6, 5), "unit": "M queries/s",
+               "cores": torch.get_num_threads(), "kind": "port",
+               "sample": f"{qn} random queries of the {n}-point room against all "
+                         f"{n} points, torch.cdist+topk in blocks of 10000, median of 3"}
+
+    line = {
+        "metric": METRIC, "value": round(value, 3), "unit": "M queries/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"synthetic indoor room, {n} fp32 points (4 cm voxel "
+                               "subsampled), queries = all points, K=16, exact kNN "
+                               "(bit-exact idx), single cloud",
+                   "points": n, "k": K, "parallelism": "single GPU"},
+        "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

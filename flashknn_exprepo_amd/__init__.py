@@ -1,0 +1,29 @@
+"""MI355X-native FlashKNN: grid kNN for point clouds on gfx950.
+
+The compute path is libflashknn.so (hand-written HIP kernels behind a C
+ABI, include/flashknn.h), loaded with ctypes; torch supplies device memory
+and streams.  Public surface:
+
+* ``FlashKNN``, ``xyz2key``, ``cal_recall``, ``vanilla_Knn`` - drop-in for
+  the reference's ``FlashKNN.functions``;
+* ``pointops.knn_query`` / ``interpolation`` / ``interpolation2`` - the
+  offset-batched layout of Pointcept's pointops;
+* ``knn`` / ``ExactKNN`` - certified exact kNN on the grid;
+* ``BatchedKNN`` - many uneven clouds, per-cloud k, one persistent launch,
+  hipGraph replay;
+* ``gathered_sqdist`` / ``idw_interpolate`` - differentiable distances and
+  inverse-distance-weighted interpolation with hand-written backward.
+"""
+from . import pointops
+from ._lib import FlashKNNError, load
+from .autograd import gathered_sqdist, idw_gather, idw_interpolate
+from .batched import BatchedKNN
+from .functions import FlashKNN, cal_recall, vanilla_Knn, xyz2key
+from .grid import Grid
+from .knn import ExactKNN, cell_sizes, knn
+
+__all__ = [
+    "FlashKNN", "xyz2key", "cal_recall", "vanilla_Knn", "pointops", "knn",
+    "ExactKNN", "cell_sizes", "BatchedKNN", "Grid", "gathered_sqdist",
+    "idw_gather", "idw_interpolate", "FlashKNNError", "load",
+]

@@ -1,0 +1,705 @@
+// C ABI of the FlashKNN gfx950 library (declared in include/flashknn.h).
+//
+// Host-side runtime: a grid handle owns the device tables of one build and
+// grows them on demand (or uses a reserved capacity for graph-capturable,
+// synchronisation-free builds).  Every entry point validates its arguments,
+// enqueues kernels on the caller's stream and returns an fk_status; a
+// message for the last failure is kept per thread (fk_last_error).
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "fk_internal.h"
+
+using namespace fk;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+int ok() {
+  g_err.clear();
+  return FK_OK;
+}
+
+#define FK_HIP(expr)                                                        \
+  do {                                                                      \
+    hipError_t e_ = (expr);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      return fail(FK_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+#define FK_CHECK(code)      \
+  do {                      \
+    int c_ = (code);        \
+    if (c_ != FK_OK) return c_; \
+  } while (0)
+
+constexpr int64_t kHardCellCap = 1ll << 28;  // dense cells per build (sync mode)
+
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
+
+template <class T>
+int grow(T *&p, int64_t &cap, int64_t need, const char *what) {
+  if (need <= cap && p) return FK_OK;
+  int64_t n = need + need / 4 + 64;
+  if (p) {
+    FK_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+  }
+  hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess)
+    return fail(FK_ERR_HIP, "hipMalloc of %s (%lld elements) failed: %s", what,
+                (long long)n, hipGetErrorString(e));
+  cap = n;
+  return FK_OK;
+}
+
+}  // namespace
+
+struct fk_grid {
+  int device = -1;
+  int32_t *scalars = nullptr;
+  int32_t *h_pinned = nullptr;
+  // per cloud
+  int64_t cap_clouds = 0;
+  int32_t *bbox = nullptr;
+  int64_t cap_bbox = 0;
+  CloudInfo *clouds = nullptr;
+  int32_t *item_prefix = nullptr;
+  int64_t cap_prefix = 0;
+  // per point
+  int64_t cap_points = 0, cap_p1 = 0, cap_p2 = 0, cap_p3 = 0, cap_p4 = 0;
+  int4 *pt_cell = nullptr;
+  int32_t *pt_lin = nullptr, *pt_rank = nullptr;
+  float4 *srec = nullptr;
+  int4 *scell = nullptr;
+  int64_t cap_scell = 0;
+  // per cell
+  int64_t cap_cells = 0, cap_c2 = 0;
+  int32_t *counts = nullptr, *cell_start = nullptr;
+  int64_t cap_qcells = 0, cap_qc2 = 0;
+  int32_t *qcounts = nullptr, *qstart = nullptr;
+  int64_t cap_part = 0;
+  int32_t *scan_part = nullptr;
+  // per query
+  int64_t cap_q = 0, cap_q1 = 0, cap_q2 = 0, cap_q3 = 0, cap_q4 = 0, cap_q5 = 0;
+  float4 *qrec = nullptr;
+  int4 *qcell = nullptr, *q_cell_abs = nullptr;
+  int32_t *qpad = nullptr, *q_lin = nullptr, *q_rank = nullptr;
+  // exact mode
+  int64_t cap_pend = 0, cap_pkeys = 0;
+  int32_t *pend_q = nullptr;
+  uint64_t *pend_keys = nullptr;
+  // host state of the last build
+  int built = 0, from_xyz = 0, num_down = 0, reserved = 0;
+  int64_t n_points = 0, n_cells = -1, n_queries = -1;
+  int32_t n_clouds = 0;
+  // captured batched step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+int bind_device(fk_grid *g) {
+  int dev = 0;
+  FK_HIP(hipGetDevice(&dev));
+  if (g->device < 0) {
+    g->device = dev;
+    FK_HIP(hipMalloc((void **)&g->scalars, sizeof(int32_t) * kNumScalars));
+    FK_HIP(hipMemset(g->scalars, 0, sizeof(int32_t) * kNumScalars));
+    FK_HIP(hipHostMalloc((void **)&g->h_pinned, sizeof(int32_t) * kNumScalars,
+                         hipHostMallocDefault));
+  } else if (dev != g->device) {
+    return fail(FK_ERR_STATE, "grid handle belongs to device %d, current is %d",
+                g->device, dev);
+  }
+  return FK_OK;
+}
+
+int ensure_clouds(fk_grid *g, int64_t B) {
+  FK_CHECK(grow(g->bbox, g->cap_bbox, 6 * B, "bbox"));
+  FK_CHECK(grow(g->clouds, g->cap_clouds, B, "clouds"));
+  FK_CHECK(grow(g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
+  return FK_OK;
+}
+
+int ensure_points(fk_grid *g, int64_t n) {
+  const int64_t m = n < 1 ? 1 : n;
+  FK_CHECK(grow(g->pt_cell, g->cap_points, m, "point cells"));
+  FK_CHECK(grow(g->pt_lin, g->cap_p1, m, "point cell ids"));
+  FK_CHECK(grow(g->pt_rank, g->cap_p2, m, "point ranks"));
+  FK_CHECK(grow(g->srec, g->cap_p3, m, "sorted records"));
+  FK_CHECK(grow(g->scell, g->cap_scell, m, "sorted cells"));
+  return FK_OK;
+}
+
+int ensure_cells(fk_grid *g, int64_t cells) {
+  FK_CHECK(grow(g->counts, g->cap_cells, cells + 1, "cell counts"));
+  FK_CHECK(grow(g->cell_start, g->cap_c2, cells + 2, "cell starts"));
+  FK_CHECK(grow(g->scan_part, g->cap_part, scan_parts_needed(cells + 2), "scan"));
+  return FK_OK;
+}
+
+int ensure_qcells(fk_grid *g, int64_t cells) {
+  FK_CHECK(grow(g->qcounts, g->cap_qcells, cells + 2, "query cell counts"));
+  FK_CHECK(grow(g->qstart, g->cap_qc2, cells + 3, "query cell starts"));
+  FK_CHECK(grow(g->scan_part, g->cap_part, scan_parts_needed(cells + 3), "scan"));
+  return FK_OK;
+}
+
+int ensure_queries(fk_grid *g, int64_t nq) {
+  const int64_t m = nq < 1 ? 1 : nq;
+  FK_CHECK(grow(g->qrec, g->cap_q, m, "query records"));
+  FK_CHECK(grow(g->qcell, g->cap_q1, m, "query cells"));
+  FK_CHECK(grow(g->q_cell_abs, g->cap_q2, m, "query cells (input order)"));
+  FK_CHECK(grow(g->qpad, g->cap_q3, m, "query pads"));
+  FK_CHECK(grow(g->q_lin, g->cap_q4, m, "query cell ids"));
+  FK_CHECK(grow(g->q_rank, g->cap_q5, m, "query ranks"));
+  return FK_OK;
+}
+
+int ensure_pending(fk_grid *g, int64_t nq, int k) {
+  const int64_t m = nq < 1 ? 1 : nq;
+  FK_CHECK(grow(g->pend_q, g->cap_pend, m, "pending queries"));
+  FK_CHECK(grow(g->pend_keys, g->cap_pkeys, m * k, "pending keys"));
+  return FK_OK;
+}
+
+BuildBuffers build_buffers(fk_grid *g) {
+  BuildBuffers b;
+  b.scalars = g->scalars;
+  b.bbox = g->bbox;
+  b.clouds = g->clouds;
+  b.pt_cell = g->pt_cell;
+  b.pt_lin = g->pt_lin;
+  b.pt_rank = g->pt_rank;
+  b.counts = g->counts;
+  b.cell_start = g->cell_start;
+  b.scan_part = g->scan_part;
+  b.srec = g->srec;
+  b.scell = g->scell;
+  b.cap_cells = g->cap_cells - 1;
+  return b;
+}
+
+__global__ void k_reset_scalars(int32_t *s) {
+  if (threadIdx.x < kNumScalars) s[threadIdx.x] = 0;
+}
+
+int read_scalars(fk_grid *g, hipStream_t s) {
+  FK_HIP(hipMemcpyAsync(g->h_pinned, g->scalars, sizeof(int32_t) * kNumScalars,
+                        hipMemcpyDeviceToHost, s));
+  FK_HIP(hipStreamSynchronize(s));
+  return FK_OK;
+}
+
+int status_message(int st) {
+  switch (st) {
+    case FK_ERR_DOMAIN:
+      return fail(st, "a batch id or cell coordinate is outside the supported "
+                      "domain (batch in [0, num_clouds), |cell| < 2^30, finite "
+                      "coordinates)");
+    case FK_ERR_GRID_TOO_LARGE:
+      return fail(st, "the dense per-cloud cell boxes exceed the cell capacity; "
+                      "use a larger cell size (or reserve more cells)");
+    default:
+      return fail(st, "device status %d", st);
+  }
+}
+
+// Common tail of both builds: layout, optional size read-back, sort.
+int finish_build(fk_grid *g, const float *xyz, const int64_t *grid, int64_t n,
+                 int32_t B, const int32_t *offsets, const float *inv_cell,
+                 int32_t flags, hipStream_t s) {
+  const bool nosync = flags & FK_BUILD_NO_SYNC;
+  BuildBuffers bb = build_buffers(g);
+  if (!nosync) bb.cap_cells = kHardCellCap;
+  FK_HIP(launch_layout(bb, B, offsets, inv_cell, s));
+  int64_t hint;
+  if (nosync) {
+    hint = g->cap_cells - 1;
+    g->n_cells = -1;
+  } else {
+    FK_CHECK(read_scalars(g, s));
+    if (g->h_pinned[kStatus] != 0) return status_message(g->h_pinned[kStatus]);
+    hint = g->h_pinned[kTotalCells];
+    FK_CHECK(ensure_cells(g, hint));
+    g->n_cells = hint;
+    bb = build_buffers(g);
+  }
+  FK_HIP(launch_sort(bb, xyz, grid, n, hint, s));
+  g->built = 1;
+  g->n_points = n;
+  g->n_clouds = B;
+  g->n_queries = -1;
+  return ok();
+}
+
+int check_stream_device(fk_grid *g) {
+  if (!g) return fail(FK_ERR_INVALID_ARG, "grid handle is NULL");
+  return bind_device(g);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fk_status_string(int status) {
+  switch (status) {
+    case FK_OK: return "ok";
+    case FK_ERR_INVALID_ARG: return "invalid argument";
+    case FK_ERR_K_RANGE: return "k out of range";
+    case FK_ERR_GRID_TOO_LARGE: return "grid too large";
+    case FK_ERR_DOMAIN: return "input outside domain";
+    case FK_ERR_HIP: return "HIP runtime error";
+    case FK_ERR_CAPACITY: return "reserved capacity exceeded";
+    case FK_ERR_STATE: return "invalid handle state";
+    default: return "unknown status";
+  }
+}
+
+const char *fk_last_error(void) { return g_err.c_str(); }
+
+int fk_abi_version(void) { return FK_ABI_VERSION; }
+
+int fk_morton_encode(const int64_t *grid_coord, const int64_t *batch_idx,
+                     int64_t n, int64_t *keys, void *stream) {
+  if (n < 0 || (n > 0 && (!grid_coord || !keys)))
+    return fail(FK_ERR_INVALID_ARG, "fk_morton_encode: bad arguments");
+  FK_HIP(launch_morton_encode(grid_coord, batch_idx, n, keys, S(stream)));
+  return ok();
+}
+
+int fk_morton_neighbor_keys(const int64_t *grid_down,
+                            const int64_t *parent_batch, int64_t num_parents,
+                            int64_t *keys, void *stream) {
+  if (num_parents < 0 || (num_parents > 0 && (!grid_down || !keys)))
+    return fail(FK_ERR_INVALID_ARG, "fk_morton_neighbor_keys: bad arguments");
+  FK_HIP(launch_morton_neighbors(grid_down, parent_batch, num_parents, keys,
+                                 S(stream)));
+  return ok();
+}
+
+int fk_grid_create(fk_grid **out) {
+  if (!out) return fail(FK_ERR_INVALID_ARG, "fk_grid_create: out is NULL");
+  *out = new fk_grid();
+  return ok();
+}
+
+int fk_grid_destroy(fk_grid *g) {
+  if (!g) return ok();
+  // Teardown keeps going past individual failures; the first is reported.
+  hipError_t first = hipSuccess;
+  auto note = [&first](hipError_t e) {
+    if (first == hipSuccess) first = e;
+  };
+  if (g->gexec) note(hipGraphExecDestroy(g->gexec));
+  if (g->graph) note(hipGraphDestroy(g->graph));
+  void *bufs[] = {g->scalars, g->bbox, g->clouds, g->item_prefix, g->pt_cell,
+                  g->pt_lin, g->pt_rank, g->srec, g->scell, g->counts,
+                  g->cell_start, g->qcounts, g->qstart, g->scan_part, g->qrec,
+                  g->qcell, g->q_cell_abs, g->qpad, g->q_lin, g->q_rank,
+                  g->pend_q, g->pend_keys};
+  for (void *p : bufs)
+    if (p) note(hipFree(p));
+  if (g->h_pinned) note(hipHostFree(g->h_pinned));
+  delete g;
+  if (first != hipSuccess)
+    return fail(FK_ERR_HIP, "fk_grid_destroy: %s", hipGetErrorString(first));
+  return ok();
+}
+
+int fk_grid_reserve(fk_grid *g, int64_t max_points, int64_t max_cells,
+                    int32_t max_clouds, int64_t max_queries) {
+  FK_CHECK(check_stream_device(g));
+  if (max_points < 0 || max_cells < 0 || max_clouds < 1 || max_queries < 0 ||
+      max_points > INT32_MAX || max_cells > kHardCellCap)
+    return fail(FK_ERR_INVALID_ARG, "fk_grid_reserve: bad sizes");
+  FK_CHECK(ensure_clouds(g, max_clouds));
+  FK_CHECK(ensure_points(g, max_points));
+  FK_CHECK(ensure_cells(g, max_cells));
+  FK_CHECK(ensure_qcells(g, max_cells));
+  const int64_t nq = max_queries > max_points ? max_queries : max_points;
+  FK_CHECK(ensure_queries(g, nq));
+  FK_CHECK(ensure_pending(g, nq, FK_MAX_K));
+  g->reserved = 1;
+  return ok();
+}
+
+int fk_grid_build_cells(fk_grid *g, const float *xyz, const int64_t *grid_coord,
+                        const int64_t *batch_idx, int64_t n, int32_t num_clouds,
+                        int32_t num_down, int32_t flags, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (n < 0 || n > INT32_MAX || num_clouds < 1 || num_down < 0 || num_down > 30 ||
+      (n > 0 && !grid_coord))
+    return fail(FK_ERR_INVALID_ARG,
+                "fk_grid_build_cells: n in [0, 2^31), num_clouds >= 1, "
+                "num_down in [0, 30] and grid_coord required");
+  hipStream_t s = S(stream);
+  const bool nosync = flags & FK_BUILD_NO_SYNC;
+  if (nosync) {
+    if (n > g->cap_points || num_clouds > g->cap_clouds || !g->counts)
+      return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs fk_grid_reserve "
+                                   "of at least n points and num_clouds clouds");
+  } else {
+    FK_CHECK(ensure_clouds(g, num_clouds));
+    FK_CHECK(ensure_points(g, n));
+    if (!g->counts) FK_CHECK(ensure_cells(g, 1024));
+  }
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, g->scalars);
+  FK_HIP(launch_build_cells(build_buffers(g), xyz, grid_coord, batch_idx, n,
+                            num_clouds, num_down, s));
+  g->from_xyz = 0;
+  g->num_down = num_down;
+  return finish_build(g, xyz, grid_coord, n, num_clouds, nullptr, nullptr, flags,
+                      s);
+}
+
+int fk_grid_build_xyz(fk_grid *g, const float *xyz, const int32_t *offsets,
+                      int32_t num_clouds, int64_t n, const float *inv_cell,
+                      int32_t flags, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (n < 0 || n > INT32_MAX || num_clouds < 1 || !offsets || !inv_cell ||
+      (n > 0 && !xyz))
+    return fail(FK_ERR_INVALID_ARG,
+                "fk_grid_build_xyz: n in [0, 2^31), num_clouds >= 1, xyz, "
+                "offsets and inv_cell required");
+  hipStream_t s = S(stream);
+  const bool nosync = flags & FK_BUILD_NO_SYNC;
+  if (nosync) {
+    if (n > g->cap_points || num_clouds > g->cap_clouds || !g->counts)
+      return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs fk_grid_reserve "
+                                   "of at least n points and num_clouds clouds");
+  } else {
+    FK_CHECK(ensure_clouds(g, num_clouds));
+    FK_CHECK(ensure_points(g, n));
+    if (!g->counts) FK_CHECK(ensure_cells(g, 1024));
+  }
+  hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, g->scalars);
+  FK_HIP(launch_build_xyz_cells(build_buffers(g), xyz, offsets, num_clouds, n,
+                                inv_cell, s));
+  g->from_xyz = 1;
+  g->num_down = 0;
+  return finish_build(g, xyz, nullptr, n, num_clouds, offsets, inv_cell, flags,
+                      s);
+}
+
+int fk_grid_attach_queries(fk_grid *g, const float *qxyz, const int64_t *qcell,
+                           const int64_t *qbatch, const int32_t *qoffsets,
+                           int64_t nq, const int32_t *qpad, int32_t flags,
+                           void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (!g->built) return fail(FK_ERR_STATE, "fk_grid_attach_queries: grid not built");
+  if (nq < 0 || nq > INT32_MAX || (nq > 0 && !qxyz))
+    return fail(FK_ERR_INVALID_ARG, "fk_grid_attach_queries: bad query arrays");
+  if (g->from_xyz) {
+    if (qcell || !qoffsets)
+      return fail(FK_ERR_INVALID_ARG, "grids built from xyz take query offsets "
+                                      "and no query cells");
+  } else if (nq > 0 && !qcell) {
+    return fail(FK_ERR_INVALID_ARG, "grids built from cells need query cells");
+  }
+  hipStream_t s = S(stream);
+  const bool nosync = flags & FK_BUILD_NO_SYNC;
+  int64_t hint = g->n_cells;
+  if (nosync || hint < 0) {
+    if (nq > g->cap_q || !g->qcounts || g->cap_qcells < g->cap_cells)
+      return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs reserved query capacity");
+    hint = g->cap_cells - 1;
+  } else {
+    FK_CHECK(ensure_queries(g, nq));
+    FK_CHECK(ensure_qcells(g, hint));
+  }
+  QueryBuffers q;
+  q.qrec = g->qrec;
+  q.qcell = g->qcell;
+  q.qpad = g->qpad;
+  q.q_lin = g->q_lin;
+  q.q_rank = g->q_rank;
+  q.q_cell_abs = g->q_cell_abs;
+  q.qcounts = g->qcounts;
+  q.qstart = g->qstart;
+  FK_HIP(launch_attach(build_buffers(g), q, qxyz, qcell, qbatch, qoffsets, nq,
+                       qpad, g->n_clouds, g->num_down, g->from_xyz, hint, s));
+  g->n_queries = nq;
+  if (!nosync) {
+    FK_CHECK(read_scalars(g, s));
+    if (g->h_pinned[kStatus] != 0) {
+      g->n_queries = -1;
+      return status_message(g->h_pinned[kStatus]);
+    }
+  }
+  return ok();
+}
+
+int fk_grid_status(fk_grid *g, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  FK_CHECK(read_scalars(g, S(stream)));
+  const int st = g->h_pinned[kStatus];
+  if (st != 0) return status_message(st);
+  return ok();
+}
+
+int fk_grid_info(fk_grid *g, int64_t *num_points, int32_t *num_clouds,
+                 int64_t *num_cells, int64_t *num_queries) {
+  if (!g) return fail(FK_ERR_INVALID_ARG, "grid handle is NULL");
+  if (num_points) *num_points = g->built ? g->n_points : -1;
+  if (num_clouds) *num_clouds = g->built ? g->n_clouds : -1;
+  if (num_cells) *num_cells = g->built ? g->n_cells : -1;
+  if (num_queries) *num_queries = g->n_queries;
+  return ok();
+}
+
+int fk_grid_tables(fk_grid *g, const void **sorted_records,
+                   const int32_t **cell_start) {
+  if (!g || !g->built) return fail(FK_ERR_STATE, "grid not built");
+  if (sorted_records) *sorted_records = g->srec;
+  if (cell_start) *cell_start = g->cell_start;
+  return ok();
+}
+
+static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
+                     float *out_dist, QueryArgs &a) {
+  if (!g || !p) return fail(FK_ERR_INVALID_ARG, "fk_query: NULL handle or params");
+  if (!g->built) return fail(FK_ERR_STATE, "fk_query: grid not built");
+  if (p->k < 1 || p->k > FK_MAX_K)
+    return fail(FK_ERR_K_RANGE, "fk_query: k = %d outside [1, %d]", p->k, FK_MAX_K);
+  if (p->mode != FK_MODE_CELLS && p->mode != FK_MODE_EXACT)
+    return fail(FK_ERR_INVALID_ARG, "fk_query: unknown mode %d", p->mode);
+  if (p->mode == FK_MODE_EXACT && !g->from_xyz)
+    return fail(FK_ERR_INVALID_ARG,
+                "FK_MODE_EXACT needs a grid built by fk_grid_build_xyz");
+  if (p->radius_cells < 1 || p->radius_cells > 1024)
+    return fail(FK_ERR_INVALID_ARG, "fk_query: radius_cells must be in [1, 1024]");
+  if (p->pad_mode < FK_PAD_SELF || p->pad_mode > FK_PAD_PER_QUERY)
+    return fail(FK_ERR_INVALID_ARG, "fk_query: unknown pad_mode");
+  if (p->dist_mode < FK_DIST_NONE || p->dist_mode > FK_DIST_SQRT)
+    return fail(FK_ERR_INVALID_ARG, "fk_query: unknown dist_mode");
+  if (p->query_set != FK_QUERIES_SELF && p->query_set != FK_QUERIES_ATTACHED)
+    return fail(FK_ERR_INVALID_ARG, "fk_query: unknown query_set");
+  if (p->query_set == FK_QUERIES_ATTACHED && g->n_queries < 0)
+    return fail(FK_ERR_STATE, "fk_query: no queries attached");
+  if (p->pad_mode == FK_PAD_PER_QUERY && p->query_set != FK_QUERIES_ATTACHED)
+    return fail(FK_ERR_INVALID_ARG, "FK_PAD_PER_QUERY needs attached queries");
+  if (p->pad_mode == FK_PAD_SELF && p->query_set != FK_QUERIES_SELF)
+    return fail(FK_ERR_INVALID_ARG, "FK_PAD_SELF is for self queries");
+  if (!out_idx) return fail(FK_ERR_INVALID_ARG, "fk_query: out_idx is NULL");
+  if (p->dist_mode != FK_DIST_NONE && !out_dist)
+    return fail(FK_ERR_INVALID_ARG, "fk_query: out_dist is NULL");
+  if (std::isnan(p->cut_r2))
+    return fail(FK_ERR_INVALID_ARG, "fk_query: cut_r2 is NaN");
+  const bool self = p->query_set == FK_QUERIES_SELF;
+  memset(&a, 0, sizeof(a));
+  a.srec = g->srec;
+  a.cell_start = g->cell_start;
+  a.clouds = g->clouds;
+  a.qrec = self ? g->srec : g->qrec;
+  a.qcell = self ? g->scell : g->qcell;
+  a.qpad = self ? nullptr : g->qpad;
+  a.nq = (int32_t)(self ? g->n_points : g->n_queries);
+  a.k = p->k;
+  a.R = p->radius_cells;
+  a.cut_r2 = p->cut_r2;
+  a.pad_mode = p->pad_mode;
+  a.dist_mode = p->dist_mode;
+  a.pad_dist = p->pad_dist;
+  a.idx64 = p->idx_64 ? 1 : 0;
+  a.out_idx = out_idx;
+  a.out_d = p->dist_mode == FK_DIST_NONE ? nullptr : out_dist;
+  a.exact = p->mode == FK_MODE_EXACT;
+  a.passes = p->passes ? (p->passes & 7) : 7;
+  a.scalars = g->scalars;
+  return FK_OK;
+}
+
+int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
+             float *out_dist, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  QueryArgs a;
+  FK_CHECK(make_args(g, p, out_idx, out_dist, a));
+  if (a.exact) {
+    if (g->reserved) {
+      if ((int64_t)a.nq > g->cap_pend || (int64_t)a.nq * a.k > g->cap_pkeys)
+        return fail(FK_ERR_CAPACITY, "fk_query: reserved pending capacity too small");
+    } else {
+      FK_CHECK(ensure_pending(g, a.nq, a.k));
+    }
+    a.pend_q = g->pend_q;
+    a.pend_keys = g->pend_keys;
+  }
+  FK_HIP(launch_query(a, S(stream)));
+  return ok();
+}
+
+int fk_query_stats(fk_grid *g, int64_t *first_pass_certified, int64_t *expanded,
+                   void *stream) {
+  FK_CHECK(check_stream_device(g));
+  FK_CHECK(read_scalars(g, S(stream)));
+  if (first_pass_certified) *first_pass_certified = g->h_pinned[kStatCertified];
+  if (expanded) *expanded = g->h_pinned[kStatExpanded];
+  return ok();
+}
+
+int fk_candidate_count(fk_grid *g, int32_t radius_cells, int32_t query_set,
+                       int64_t *total, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (!total) return fail(FK_ERR_INVALID_ARG, "fk_candidate_count: total is NULL");
+  fk_query_params p;
+  memset(&p, 0, sizeof(p));
+  p.k = 1;
+  p.mode = FK_MODE_CELLS;
+  p.radius_cells = radius_cells;
+  p.cut_r2 = INFINITY;
+  p.pad_mode = query_set == FK_QUERIES_SELF ? FK_PAD_SELF : FK_PAD_MINUS1;
+  p.query_set = query_set;
+  QueryArgs a;
+  int dummy = 0;
+  FK_CHECK(make_args(g, &p, &dummy, nullptr, a));
+  hipStream_t s = S(stream);
+  unsigned long long *acc = (unsigned long long *)(g->scalars + 8);
+  FK_HIP(hipMemsetAsync(acc, 0, sizeof(unsigned long long), s));
+  FK_HIP(launch_candidate_count(a, acc, s));
+  unsigned long long h = 0;
+  FK_HIP(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+  FK_HIP(hipStreamSynchronize(s));
+  *total = (int64_t)h;
+  return ok();
+}
+
+static int batched_args(fk_grid *g, const fk_query_params *p,
+                        const int32_t *k_per_cloud, const int64_t *out_offsets,
+                        void *out_idx, float *out_dist, QueryArgs &a) {
+  FK_CHECK(make_args(g, p, out_idx, out_dist, a));
+  if (!g->from_xyz)
+    return fail(FK_ERR_INVALID_ARG, "fk_batched_query needs a grid built by "
+                                    "fk_grid_build_xyz (cloud offsets)");
+  if (p->query_set != FK_QUERIES_SELF)
+    return fail(FK_ERR_INVALID_ARG, "fk_batched_query runs self queries");
+  if (!k_per_cloud || !out_offsets)
+    return fail(FK_ERR_INVALID_ARG, "fk_batched_query: k_per_cloud and "
+                                    "out_offsets required");
+  if (a.exact) {
+    if ((int64_t)a.nq > g->cap_pend || (int64_t)a.nq * a.k > g->cap_pkeys) {
+      if (g->reserved)
+        return fail(FK_ERR_CAPACITY, "reserved pending capacity too small");
+      FK_CHECK(ensure_pending(g, a.nq, a.k));
+    }
+    a.pend_q = g->pend_q;
+    a.pend_keys = g->pend_keys;
+  }
+  a.k_per_cloud = k_per_cloud;
+  a.out_offsets = out_offsets;
+  a.num_clouds = g->n_clouds;
+  a.item_cloud = g->item_prefix;
+  return FK_OK;
+}
+
+int fk_batched_query(fk_grid *g, const fk_query_params *p,
+                     const int32_t *k_per_cloud, const int64_t *out_offsets,
+                     void *out_idx, float *out_dist, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  QueryArgs a;
+  FK_CHECK(batched_args(g, p, k_per_cloud, out_offsets, out_idx, out_dist, a));
+  hipStream_t s = S(stream);
+  FK_HIP(launch_items(g->clouds, g->cell_start, g->n_clouds, g->item_prefix,
+                      g->scalars, s));
+  FK_HIP(launch_batched(a, p->k, s));
+  return ok();
+}
+
+int fk_batched_capture(fk_grid *g, const fk_query_params *p,
+                       const int32_t *k_per_cloud, const int64_t *out_offsets,
+                       void *out_idx, float *out_dist, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (!stream)
+    return fail(FK_ERR_INVALID_ARG, "fk_batched_capture needs a non-null stream");
+  QueryArgs a;
+  FK_CHECK(batched_args(g, p, k_per_cloud, out_offsets, out_idx, out_dist, a));
+  if (g->gexec) {
+    FK_HIP(hipGraphExecDestroy(g->gexec));
+    g->gexec = nullptr;
+  }
+  if (g->graph) {
+    FK_HIP(hipGraphDestroy(g->graph));
+    g->graph = nullptr;
+  }
+  hipStream_t s = S(stream);
+  FK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  hipError_t e1 = launch_items(g->clouds, g->cell_start, g->n_clouds,
+                               g->item_prefix, g->scalars, s);
+  hipError_t e2 = e1 == hipSuccess ? launch_batched(a, p->k, s) : e1;
+  hipGraph_t graph = nullptr;
+  hipError_t e3 = hipStreamEndCapture(s, &graph);
+  if (e2 != hipSuccess || e3 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(FK_ERR_HIP, "graph capture failed: %s",
+                hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+  }
+  g->graph = graph;
+  FK_HIP(hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0));
+  return ok();
+}
+
+int fk_graph_replay(fk_grid *g, void *stream) {
+  if (!g || !g->gexec) return fail(FK_ERR_STATE, "fk_graph_replay: nothing captured");
+  FK_HIP(hipGraphLaunch(g->gexec, S(stream)));
+  return ok();
+}
+
+int64_t fk_backward_workspace_bytes(int64_t nq, int64_t ns, int32_t k) {
+  if (nq < 0 || ns < 0 || k < 1) return -1;
+  return backward_workspace_bytes(nq, ns, k);
+}
+
+int fk_dist_backward(const float *q, int64_t nq, const float *p, int64_t ns,
+                     const int64_t *idx, int32_t k, const float *grad_d,
+                     float *grad_q, float *grad_p, void *workspace,
+                     void *stream) {
+  if (nq < 0 || ns < 0 || k < 1 || nq * (int64_t)k > INT32_MAX ||
+      (nq > 0 && (!q || !idx || !grad_d)) || (ns > 0 && !p) ||
+      (grad_p && ns > 0 && !workspace))
+    return fail(FK_ERR_INVALID_ARG, "fk_dist_backward: bad arguments");
+  FK_HIP(launch_dist_backward(q, nq, p, ns, idx, k, grad_d, grad_q, grad_p,
+                              workspace, S(stream)));
+  return ok();
+}
+
+int fk_idw_forward(const float *feat, int64_t ns, int32_t channels,
+                   const int64_t *idx, const float *w, int64_t nq, int32_t k,
+                   float *out, void *stream) {
+  if (ns < 0 || nq < 0 || k < 1 || channels < 1 ||
+      (nq > 0 && (!feat || !idx || !w || !out)))
+    return fail(FK_ERR_INVALID_ARG, "fk_idw_forward: bad arguments");
+  FK_HIP(launch_idw_forward(feat, ns, channels, idx, w, nq, k, out, S(stream)));
+  return ok();
+}
+
+int fk_idw_backward(const float *feat, int64_t ns, int32_t channels,
+                    const int64_t *idx, const float *w, int64_t nq, int32_t k,
+                    const float *grad_out, float *grad_feat, float *grad_w,
+                    void *workspace, void *stream) {
+  if (ns < 0 || nq < 0 || k < 1 || channels < 1 || nq * (int64_t)k > INT32_MAX ||
+      (nq > 0 && (!idx || !w || !grad_out)) || (grad_w && !feat) ||
+      (grad_feat && ns > 0 && !workspace))
+    return fail(FK_ERR_INVALID_ARG, "fk_idw_backward: bad arguments");
+  FK_HIP(launch_idw_backward(feat, ns, channels, idx, w, nq, k, grad_out,
+                             grad_feat, grad_w, workspace, S(stream)));
+  return ok();
+}
+
+}  // extern "C"

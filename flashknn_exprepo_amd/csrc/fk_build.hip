@@ -1,0 +1,610 @@
+// Grid construction for gfx950: cell ids, per-cloud dense cell boxes,
+// counting sort of points by cell, cell start table, sorted records.
+//
+// Replaces the torch construction of FlashKNN.query
+// (FlashKNN/functions/FlashKnnWrapper.py:226-337: Morton key, argsort,
+// unique_consecutive, searchsorted neighbour table, cumsum candidate table).
+// Instead of a Morton sort plus a searched 27-neighbour table, every cloud
+// gets a dense box of cells laid out x-major, z-fastest: the 27-cell
+// neighbourhood of a cell is 9 column segments, each one contiguous run of
+// the cell-sorted points, found by two reads of the cell start table.  No
+// host synchronisation is needed when the cell capacity is reserved.
+#include "fk_internal.h"
+
+#include <climits>
+
+namespace fk {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kBlock * kScanItems;  // 4096 elements per block
+
+__device__ __forceinline__ bool wave_all(bool p) {
+  return __ballot(!p) == 0ull;  // inactive lanes count as true
+}
+
+__global__ void k_fill_i32(int32_t *p, int32_t v, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Fill p[0, *n_dev + extra) with v (size read on the device).
+__global__ void k_fill_dev(int32_t *p, int32_t v, const int32_t *n_dev,
+                           int32_t extra) {
+  const int64_t n = (int64_t)(*n_dev) + extra;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// ------------------------------------------------------------------ Morton
+
+// Bit spreading by the standard magic-number interleave: bit i of v moves to
+// bit 3i.
+__device__ __forceinline__ uint64_t spread3(uint64_t v) {
+  v &= 0xffffull;
+  v = (v | (v << 16)) & 0x0000ff0000ffull;
+  v = (v | (v << 8)) & 0x00f00f00f00full;
+  v = (v | (v << 4)) & 0x0c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x249249249249ull;
+  return v;
+}
+
+__device__ __forceinline__ int64_t morton(int64_t x, int64_t y, int64_t z,
+                                          int64_t b) {
+  return (int64_t)((spread3((uint64_t)x) << 2) | (spread3((uint64_t)y) << 1) |
+                   spread3((uint64_t)z) | ((uint64_t)b << 48));
+}
+
+__global__ void k_morton_encode(const int64_t *grid, const int64_t *batch,
+                                int64_t n, int64_t *keys) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = morton(grid[3 * i], grid[3 * i + 1], grid[3 * i + 2],
+                   batch ? batch[i] : 0);
+}
+
+__global__ void k_morton_neighbors(const int64_t *gd, const int64_t *pb,
+                                   int64_t P, int64_t *keys) {
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t >= P * 27) return;
+  const int64_t p = t / 27;
+  const int j = (int)(t - p * 27);
+  // offsets [0,-1,1] per axis, x slowest (ij meshgrid order, centre first)
+  const int ox = j / 9, oy = (j / 3) % 3, oz = j % 3;
+  auto off = [](int s) { return s == 0 ? 0 : (s == 1 ? -1 : 1); };
+  keys[t] = morton(gd[3 * p] + off(ox), gd[3 * p + 1] + off(oy),
+                   gd[3 * p + 2] + off(oz), pb ? pb[p] : 0);
+}
+
+// ---------------------------------------------------------- cells + bbox
+
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// Per-cloud bounding box of cell coordinates.  When all valid lanes of the
+// wave belong to one cloud (the usual case) the wave reduces first and one
+// lane issues the six atomics.
+__device__ void bbox_update(int32_t *bbox, int cloud, int cx, int cy, int cz,
+                            bool valid) {
+  const uint64_t vm = __ballot(valid);
+  if (vm == 0ull) return;
+  const int leader = __builtin_ctzll(vm);
+  const int c0 = __shfl(cloud, leader, kWave);
+  if (wave_all(!valid || cloud == c0)) {
+    const int lx = wave_min(valid ? cx : INT_MAX), hx = wave_max(valid ? cx : INT_MIN);
+    const int ly = wave_min(valid ? cy : INT_MAX), hy = wave_max(valid ? cy : INT_MIN);
+    const int lz = wave_min(valid ? cz : INT_MAX), hz = wave_max(valid ? cz : INT_MIN);
+    if ((int)(threadIdx.x & (kWave - 1)) == leader) {
+      int32_t *b = bbox + 6 * c0;
+      atomicMin(b + 0, lx); atomicMin(b + 1, ly); atomicMin(b + 2, lz);
+      atomicMax(b + 3, hx); atomicMax(b + 4, hy); atomicMax(b + 5, hz);
+    }
+  } else if (valid) {
+    int32_t *b = bbox + 6 * cloud;
+    atomicMin(b + 0, cx); atomicMin(b + 1, cy); atomicMin(b + 2, cz);
+    atomicMax(b + 3, cx); atomicMax(b + 4, cy); atomicMax(b + 5, cz);
+  }
+}
+
+__global__ void k_init_bbox(int32_t *bbox, int32_t B) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B) return;
+  bbox[6 * c + 0] = bbox[6 * c + 1] = bbox[6 * c + 2] = INT_MAX;
+  bbox[6 * c + 3] = bbox[6 * c + 4] = bbox[6 * c + 5] = INT_MIN;
+}
+
+constexpr int64_t kCellLimit = 1ll << 30;  // |cell coordinate| bound
+
+__device__ __forceinline__ bool cell_ok(int64_t v) {
+  return v > -kCellLimit && v < kCellLimit;
+}
+
+// cell = grid >> num_down (arithmetic shift = floor division by 2^num_down)
+__global__ void k_cells_from_grid(const int64_t *grid, const int64_t *batch,
+                                  int64_t n, int32_t num_down, int32_t B,
+                                  int4 *pt_cell, int32_t *bbox,
+                                  int32_t *scalars) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool valid = i < n;
+  int cx = 0, cy = 0, cz = 0, cl = 0;
+  if (valid) {
+    const int64_t gx = grid[3 * i] >> num_down, gy = grid[3 * i + 1] >> num_down,
+                  gz = grid[3 * i + 2] >> num_down;
+    const int64_t b = batch ? batch[i] : 0;
+    if (b < 0 || b >= B || !cell_ok(gx) || !cell_ok(gy) || !cell_ok(gz)) {
+      raise_status(scalars, FK_ERR_DOMAIN);
+      valid = false;
+      pt_cell[i] = make_int4(0, 0, 0, -1);
+    } else {
+      cx = (int)gx; cy = (int)gy; cz = (int)gz; cl = (int)b;
+      pt_cell[i] = make_int4(cx, cy, cz, cl);
+    }
+  }
+  bbox_update(bbox, cl, cx, cy, cz, valid);
+}
+
+// first c with offsets[c] > i  (offsets = cumulative segment ends)
+__device__ __forceinline__ int cloud_of(const int32_t *offsets, int B,
+                                        int64_t i) {
+  int lo = 0, hi = B;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if ((int64_t)offsets[mid] > i) hi = mid; else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Scaled coordinate u = x * inv (one rounding, no contraction possible);
+// the query kernels recompute it identically for certification.
+__device__ __forceinline__ float scaled(float x, float inv) {
+  return __fmul_rn(x, inv);
+}
+
+__global__ void k_cells_from_xyz(const float *xyz, const int32_t *offsets,
+                                 int32_t B, int64_t n, const float *inv_cell,
+                                 int4 *pt_cell, int32_t *bbox,
+                                 int32_t *scalars) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  bool valid = i < n;
+  int cx = 0, cy = 0, cz = 0, cl = 0;
+  if (valid) {
+    cl = cloud_of(offsets, B, i);
+    bool ok = cl < B;
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    if (ok) {
+      const float inv = inv_cell[cl];
+      fx = floorf(scaled(xyz[3 * i], inv));
+      fy = floorf(scaled(xyz[3 * i + 1], inv));
+      fz = floorf(scaled(xyz[3 * i + 2], inv));
+      // rejects NaN as well
+      ok = fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
+           fabsf(fz) < (float)kCellLimit;
+    }
+    if (!ok) {
+      raise_status(scalars, FK_ERR_DOMAIN);
+      valid = false;
+      cl = 0;
+      pt_cell[i] = make_int4(0, 0, 0, -1);
+    } else {
+      cx = (int)fx; cy = (int)fy; cz = (int)fz;
+      pt_cell[i] = make_int4(cx, cy, cz, cl);
+    }
+  }
+  bbox_update(bbox, cl, cx, cy, cz, valid);
+}
+
+// One block: per-cloud box sizes, exclusive scan of cell counts, CloudInfo.
+__global__ void __launch_bounds__(1024)
+k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
+         const float *inv_cell, int64_t cap_cells, CloudInfo *clouds,
+         int32_t *scalars) {
+  __shared__ int64_t warp_sums[16];
+  __shared__ int64_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < B; c0 += 1024) {
+    const int c = c0 + threadIdx.x;
+    int64_t cells = 0;
+    int nx = 0, ny = 0, nz = 0, mnx = 0, mny = 0, mnz = 0;
+    if (c < B) {
+      const int32_t *b = bbox + 6 * c;
+      if (b[0] <= b[3]) {
+        mnx = b[0]; mny = b[1]; mnz = b[2];
+        const int64_t ex = (int64_t)b[3] - b[0] + 1, ey = (int64_t)b[4] - b[1] + 1,
+                      ez = (int64_t)b[5] - b[2] + 1;
+        if (ex * ey * ez > kCellLimit) {
+          raise_status(scalars, FK_ERR_GRID_TOO_LARGE);
+        } else {
+          nx = (int)ex; ny = (int)ey; nz = (int)ez;
+          cells = ex * ey * ez;
+        }
+      }
+    }
+    // inclusive wave scan
+    int64_t v = cells;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t u = __shfl_up(v, o, kWave);
+      if (lane >= o) v += u;
+    }
+    if (lane == 63) warp_sums[w] = v;
+    __syncthreads();
+    int64_t before = carry;
+    for (int j = 0; j < w; ++j) before += warp_sums[j];
+    const int64_t excl = before + v - cells;
+    if (c < B) {
+      CloudInfo ci;
+      ci.base = (int32_t)(excl < INT_MAX ? excl : INT_MAX);
+      ci.nx = nx; ci.ny = ny; ci.nz = nz;
+      ci.minx = mnx; ci.miny = mny; ci.minz = mnz;
+      ci.first_id = offsets ? (c ? offsets[c - 1] : 0) : 0;
+      ci.num_points = offsets ? offsets[c] - ci.first_id : 0;
+      ci.inv_cell = inv_cell ? inv_cell[c] : 0.f;
+      ci.pad0 = ci.pad1 = 0;
+      clouds[c] = ci;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + cells;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int64_t total = carry;
+    if (total > cap_cells) {
+      raise_status(scalars, FK_ERR_GRID_TOO_LARGE);
+      scalars[kTotalCells] = (int32_t)(cap_cells < INT_MAX ? cap_cells : INT_MAX);
+    } else {
+      scalars[kTotalCells] = (int32_t)total;
+    }
+  }
+}
+
+__device__ __forceinline__ int32_t lin_of(const CloudInfo &ci, int lx, int ly,
+                                          int lz) {
+  return ci.base + (lx * ci.ny + ly) * ci.nz + lz;
+}
+
+// Linear cell id and in-cell rank (atomic counting) of every point.
+__global__ void k_count(const int4 *pt_cell, int64_t n, const CloudInfo *clouds,
+                        int32_t *counts, int32_t *pt_lin, int32_t *pt_rank,
+                        const int32_t *scalars) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n || scalars[kStatus] != 0) return;
+  const int4 c = pt_cell[i];
+  const CloudInfo ci = clouds[c.w];
+  const int32_t lin = lin_of(ci, c.x - ci.minx, c.y - ci.miny, c.z - ci.minz);
+  pt_lin[i] = lin;
+  pt_rank[i] = atomicAdd(counts + lin, 1);
+}
+
+// ------------------------------------------------------------------- scan
+// Exclusive scan out[0..N] of in[0..N), N = *n_dev + extra, out[N] = total.
+
+__device__ __forceinline__ int block_reduce_sum(int v, int *red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  int s = 0;
+  for (int j = 0; j < (int)(blockDim.x >> 6); ++j) s += red[j];
+  __syncthreads();
+  return s;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_scan_reduce(const int32_t *in, const int32_t *n_dev, int64_t n_fixed,
+              int32_t extra, int32_t *part) {
+  __shared__ int red[kBlock / 64];
+  const int64_t N = (n_dev ? (int64_t)(*n_dev) : n_fixed) + extra;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int64_t idx = base + threadIdx.x + j * kBlock;
+    if (idx < N) s += in[idx];
+  }
+  s = block_reduce_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(1024)
+k_scan_partials(int32_t *part, int32_t nparts) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < nparts; c0 += 1024) {
+    const int c = c0 + threadIdx.x;
+    const int x = c < nparts ? part[c] : 0;
+    int v = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(v, o, kWave);
+      if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    int before = carry;
+    for (int j = 0; j < w; ++j) before += wsum[j];
+    if (c < nparts) part[c] = before + v - x;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = before + v;
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_scan_apply(const int32_t *in, const int32_t *n_dev, int64_t n_fixed,
+             int32_t extra, const int32_t *part, int32_t *out) {
+  __shared__ int wsum[kBlock / 64];
+  const int64_t N = (n_dev ? (int64_t)(*n_dev) : n_fixed) + extra;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int s = 0;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int64_t idx = base + j;
+    v[j] = idx < N ? in[idx] : 0;
+    s += v[j];
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = s;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(incl, o, kWave);
+    if (lane >= o) incl += u;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  int before = part[blockIdx.x];
+  for (int j = 0; j < w; ++j) before += wsum[j];
+  int run = before + incl - s;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int64_t idx = base + j;
+    if (idx <= N) out[idx] = run;
+    run += v[j];
+  }
+}
+
+// ---------------------------------------------------------------- scatter
+
+__global__ void k_scatter(const float *xyz, const int64_t *grid, int64_t n,
+                          const int4 *pt_cell, const int32_t *pt_lin,
+                          const int32_t *pt_rank, const int32_t *cell_start,
+                          const CloudInfo *clouds, float4 *srec, int4 *scell,
+                          const int32_t *scalars) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n || scalars[kStatus] != 0) return;
+  const int4 c = pt_cell[i];
+  const CloudInfo ci = clouds[c.w];
+  const int32_t pos = cell_start[pt_lin[i]] + pt_rank[i];
+  float x, y, z;
+  if (xyz) {
+    x = xyz[3 * i]; y = xyz[3 * i + 1]; z = xyz[3 * i + 2];
+  } else {
+    x = (float)grid[3 * i]; y = (float)grid[3 * i + 1]; z = (float)grid[3 * i + 2];
+  }
+  srec[pos] = make_float4(x, y, z, __int_as_float((int)i));
+  scell[pos] = make_int4(c.x - ci.minx, c.y - ci.miny, c.z - ci.minz, c.w);
+}
+
+// ---------------------------------------------------------------- queries
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) {
+  return v < lo ? lo : (v > hi ? hi : v);
+}
+
+__device__ __forceinline__ int sat_local(int64_t v) {
+  return (int)(v < -kCellLimit ? -kCellLimit : (v > kCellLimit ? kCellLimit : v));
+}
+
+// Query cell in the grid's local cell space, its sort bucket (the cell,
+// clamped into the box; `total` for clouds without cells) and rank.
+__global__ void k_qcells(const float *qxyz, const int64_t *qcell,
+                         const int64_t *qbatch, const int32_t *qoffsets,
+                         int64_t nq, int32_t B, int32_t num_down,
+                         int32_t from_xyz, const CloudInfo *clouds,
+                         int32_t *qcounts, int32_t *q_lin, int32_t *q_rank,
+                         int4 *q_cell, int32_t *scalars) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nq || scalars[kStatus] != 0) return;
+  int cl;
+  int64_t ax, ay, az;
+  if (from_xyz) {
+    cl = cloud_of(qoffsets, B, i);
+    if (cl >= B) { raise_status(scalars, FK_ERR_DOMAIN); return; }
+    const float inv = clouds[cl].inv_cell;
+    const float fx = floorf(scaled(qxyz[3 * i], inv));
+    const float fy = floorf(scaled(qxyz[3 * i + 1], inv));
+    const float fz = floorf(scaled(qxyz[3 * i + 2], inv));
+    if (!(fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
+          fabsf(fz) < (float)kCellLimit)) {
+      raise_status(scalars, FK_ERR_DOMAIN);
+      return;
+    }
+    ax = (int64_t)fx; ay = (int64_t)fy; az = (int64_t)fz;
+  } else {
+    const int64_t b = qbatch ? qbatch[i] : 0;
+    if (b < 0 || b >= B) { raise_status(scalars, FK_ERR_DOMAIN); return; }
+    cl = (int)b;
+    ax = qcell[3 * i] >> num_down;
+    ay = qcell[3 * i + 1] >> num_down;
+    az = qcell[3 * i + 2] >> num_down;
+  }
+  const CloudInfo ci = clouds[cl];
+  const int lx = sat_local(ax - ci.minx), ly = sat_local(ay - ci.miny),
+            lz = sat_local(az - ci.minz);
+  q_cell[i] = make_int4(lx, ly, lz, cl);
+  int32_t lin;
+  if (ci.nx == 0)
+    lin = scalars[kTotalCells];
+  else
+    lin = lin_of(ci, clampi(lx, 0, ci.nx - 1), clampi(ly, 0, ci.ny - 1),
+                 clampi(lz, 0, ci.nz - 1));
+  q_lin[i] = lin;
+  q_rank[i] = atomicAdd(qcounts + lin, 1);
+}
+
+__global__ void k_qscatter(const float *qxyz, int64_t nq, const int32_t *q_lin,
+                           const int32_t *q_rank, const int4 *q_cell,
+                           const int32_t *qstart, const int32_t *qpad_in,
+                           float4 *qrec, int4 *qcell, int32_t *qpad,
+                           const int32_t *scalars) {
+  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= nq || scalars[kStatus] != 0) return;
+  const int32_t pos = qstart[q_lin[i]] + q_rank[i];
+  qrec[pos] = make_float4(qxyz[3 * i], qxyz[3 * i + 1], qxyz[3 * i + 2],
+                          __int_as_float((int)i));
+  qcell[pos] = q_cell[i];
+  qpad[pos] = qpad_in ? qpad_in[i] : -1;
+}
+
+inline unsigned blocks_for(int64_t n, int per = kBlock) {
+  int64_t b = (n + per - 1) / per;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+hipError_t launch_scan(const int32_t *in, int32_t *out, const int32_t *n_dev,
+                       int32_t extra, int64_t n_hint, int32_t *part,
+                       hipStream_t s) {
+  const unsigned nb = blocks_for(n_hint + extra + 1, kScanTile);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(kBlock), 0, s, in, n_dev,
+                     n_hint, extra, part);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(1024), 0, s, part,
+                     (int32_t)nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3(nb), dim3(kBlock), 0, s, in, n_dev,
+                     n_hint, extra, part, out);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int64_t scan_parts_needed(int64_t n) { return (n + 2 + kScanTile - 1) / kScanTile + 1; }
+
+// Exclusive scan of a host-sized array: out[0..n] (out[n] = total).
+hipError_t launch_scan_host(const int32_t *in, int32_t *out, int64_t n,
+                            int32_t *part, hipStream_t s) {
+  return launch_scan(in, out, nullptr, 0, n, part, s);
+}
+
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  unsigned nb = blocks_for(n);
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(k_fill_i32, dim3(nb), dim3(kBlock), 0, s, p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_morton_encode(const int64_t *grid, const int64_t *batch,
+                                int64_t n, int64_t *keys, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton_encode, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                     grid, batch, n, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_morton_neighbors(const int64_t *gd, const int64_t *pb,
+                                   int64_t P, int64_t *keys, hipStream_t s) {
+  if (P <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_morton_neighbors, dim3(blocks_for(P * 27)),
+                     dim3(kBlock), 0, s, gd, pb, P, keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_cells(const BuildBuffers &b, const float *xyz,
+                              const int64_t *grid, const int64_t *batch,
+                              int64_t n, int32_t B, int32_t num_down,
+                              hipStream_t s) {
+  (void)xyz;
+  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s,
+                     b.bbox, B);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cells_from_grid, dim3(blocks_for(n)), dim3(kBlock), 0,
+                       s, grid, batch, n, num_down, B, b.pt_cell, b.bbox,
+                       b.scalars);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_xyz_cells(const BuildBuffers &b, const float *xyz,
+                                  const int32_t *offsets, int32_t B, int64_t n,
+                                  const float *inv_cell, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s,
+                     b.bbox, B);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cells_from_xyz, dim3(blocks_for(n)), dim3(kBlock), 0,
+                       s, xyz, offsets, B, n, inv_cell, b.pt_cell, b.bbox,
+                       b.scalars);
+  return hipGetLastError();
+}
+
+hipError_t launch_layout(const BuildBuffers &b, int32_t B,
+                         const int32_t *offsets, const float *inv_cell,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, s, b.bbox, B, offsets,
+                     inv_cell, b.cap_cells, b.clouds, b.scalars);
+  return hipGetLastError();
+}
+
+hipError_t launch_sort(const BuildBuffers &b, const float *xyz,
+                       const int64_t *grid, int64_t n, int64_t cells_hint,
+                       hipStream_t s) {
+  unsigned nfill = blocks_for(cells_hint + 1);
+  if (nfill > 8192) nfill = 8192;
+  hipLaunchKernelGGL(k_fill_dev, dim3(nfill), dim3(kBlock), 0, s, b.counts, 0,
+                     b.scalars + kTotalCells, 0);
+  if (n > 0)
+    hipLaunchKernelGGL(k_count, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+                       b.pt_cell, n, b.clouds, b.counts, b.pt_lin, b.pt_rank,
+                       b.scalars);
+  hipError_t e = launch_scan(b.counts, b.cell_start, b.scalars + kTotalCells, 0,
+                             cells_hint, b.scan_part, s);
+  if (e != hipSuccess) return e;
+  if (n > 0)
+    hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s, xyz,
+                       grid, n, b.pt_cell, b.pt_lin, b.pt_rank, b.cell_start,
+                       b.clouds, b.srec, b.scell, b.scalars);
+  return hipGetLastError();
+}
+
+hipError_t launch_attach(const BuildBuffers &b, const QueryBuffers &q,
+                         const float *qxyz, const int64_t *qcell,
+                         const int64_t *qbatch, const int32_t *qoffsets,
+                         int64_t nq, const int32_t *qpad, int32_t B,
+                         int32_t num_down, int from_xyz, int64_t cells_hint,
+                         hipStream_t s) {
+  unsigned nfill = blocks_for(cells_hint + 2);
+  if (nfill > 8192) nfill = 8192;
+  // one extra bucket (index total) for queries of clouds without cells
+  hipLaunchKernelGGL(k_fill_dev, dim3(nfill), dim3(kBlock), 0, s, q.qcounts, 0,
+                     b.scalars + kTotalCells, 1);
+  if (nq > 0)
+    hipLaunchKernelGGL(k_qcells, dim3(blocks_for(nq)), dim3(kBlock), 0, s, qxyz,
+                       qcell, qbatch, qoffsets, nq, B, num_down, from_xyz,
+                       b.clouds, q.qcounts, q.q_lin, q.q_rank, q.q_cell_abs,
+                       b.scalars);
+  hipError_t e = launch_scan(q.qcounts, q.qstart, b.scalars + kTotalCells, 1,
+                             cells_hint, b.scan_part, s);
+  if (e != hipSuccess) return e;
+  if (nq > 0)
+    hipLaunchKernelGGL(k_qscatter, dim3(blocks_for(nq)), dim3(kBlock), 0, s,
+                       qxyz, nq, q.q_lin, q.q_rank, q.q_cell_abs, q.qstart,
+                       qpad, q.qrec, q.qcell, q.qpad, b.scalars);
+  return hipGetLastError();
+}
+
+}  // namespace fk

@@ -1,0 +1,169 @@
+// Internal declarations shared by the FlashKNN HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "flashknn.h"
+
+namespace fk {
+
+constexpr int kWave = 64;
+
+// Slots of the small device scalar block owned by every grid handle.
+enum Scalar : int {
+  kStatus = 0,        // first error raised by a kernel (fk_status), 0 = ok
+  kTotalCells = 1,    // cells of all per-cloud boxes
+  kPending = 2,       // exact mode: queries left uncertified by pass 1
+  kQueueHead = 3,     // batched mode: next work item
+  kStatCertified = 4, // exact mode: certified by pass 1
+  kStatExpanded = 5,  // exact mode: resolved by shell expansion
+  kPending2 = 6,      // batched exact mode: pending list of pass 2 head
+  kNumScalars = 16
+};
+
+// Dense cell box of one cloud.  Cell (lx,ly,lz) of cloud c (local
+// coordinates, 0 <= l < n) has linear id base + (lx*ny + ly)*nz + lz, so the
+// three cells of a column segment (x, y, z-1..z+1) are adjacent and their
+// points form one contiguous run of the cell-sorted arrays.
+struct CloudInfo {
+  int32_t base;            // first linear cell id of the box
+  int32_t nx, ny, nz;      // box size in cells (0 => empty cloud)
+  int32_t minx, miny, minz;// absolute cell coordinate of local (0,0,0)
+  int32_t first_id;        // offsets-built grids: first point id of the cloud
+  float inv_cell;          // xyz-built grids: 1 / cell size (0 otherwise)
+  int32_t num_points;      // points of the cloud
+  int32_t pad0, pad1;
+};
+static_assert(sizeof(CloudInfo) == 48, "CloudInfo layout");
+
+__device__ __forceinline__ void raise_status(int32_t *scalars, int code) {
+  atomicCAS(scalars + kStatus, 0, code);
+}
+
+__device__ __forceinline__ int xcd_remap(int bid, int nb) {
+  // Workgroups are dispatched round-robin over the 8 XCDs (bid % 8).  Give
+  // each XCD one contiguous range of logical tiles so that neighbouring
+  // tiles, which share candidate cells, share that XCD's L2.
+  const int x = bid & 7, l = bid >> 3, per = nb >> 3, rem = nb & 7;
+  return x * per + (x < rem ? x : rem) + l;
+}
+
+// --------------------------------------------------------------- launches
+// (defined in fk_build.hip / fk_query.hip / fk_backward.hip)
+
+hipError_t launch_fill_i32(int32_t *p, int32_t v, int64_t n, hipStream_t s);
+hipError_t launch_morton_encode(const int64_t *grid, const int64_t *batch,
+                                int64_t n, int64_t *keys, hipStream_t s);
+hipError_t launch_morton_neighbors(const int64_t *grid_down,
+                                   const int64_t *pbatch, int64_t P,
+                                   int64_t *keys, hipStream_t s);
+
+struct BuildBuffers {
+  int32_t *scalars;      // [kNumScalars]
+  int32_t *bbox;         // [6 * clouds]
+  CloudInfo *clouds;     // [clouds]
+  int4 *pt_cell;         // [n]   absolute cell + cloud of input points
+  int32_t *pt_lin;       // [n]   linear cell id of input points
+  int32_t *pt_rank;      // [n]   rank inside the cell (atomic order)
+  int32_t *counts;       // [cap_cells]
+  int32_t *cell_start;   // [cap_cells + 1]
+  int32_t *scan_part;    // [scan blocks]
+  float4 *srec;          // [n]   sorted (x, y, z, id bits)
+  int4 *scell;           // [n]   sorted local cell + cloud
+  int64_t cap_cells;
+};
+
+hipError_t launch_build_cells(const BuildBuffers &b, const float *xyz,
+                              const int64_t *grid, const int64_t *batch,
+                              int64_t n, int32_t num_clouds, int32_t num_down,
+                              hipStream_t s);
+hipError_t launch_build_xyz_cells(const BuildBuffers &b, const float *xyz,
+                                  const int32_t *offsets, int32_t num_clouds,
+                                  int64_t n, const float *inv_cell,
+                                  hipStream_t s);
+hipError_t launch_layout(const BuildBuffers &b, int32_t num_clouds,
+                         const int32_t *offsets, const float *inv_cell,
+                         hipStream_t s);
+// count / scan / scatter; cells_hint = host upper bound of total cells
+hipError_t launch_sort(const BuildBuffers &b, const float *xyz,
+                       const int64_t *grid, int64_t n, int64_t cells_hint,
+                       hipStream_t s);
+
+struct QueryBuffers {
+  float4 *qrec;          // [nq] sorted (x, y, z, output row bits)
+  int4 *qcell;           // [nq] sorted local cell + cloud
+  int32_t *qpad;         // [nq] sorted per-query pad value
+  int32_t *q_lin;        // [nq]
+  int32_t *q_rank;       // [nq]
+  int4 *q_cell_abs;      // [nq]
+  int32_t *qcounts;      // [cap_cells]
+  int32_t *qstart;       // [cap_cells + 1]
+};
+
+hipError_t launch_attach(const BuildBuffers &b, const QueryBuffers &q,
+                         const float *qxyz, const int64_t *qcell,
+                         const int64_t *qbatch, const int32_t *qoffsets,
+                         int64_t nq, const int32_t *qpad, int32_t num_clouds,
+                         int32_t num_down, int from_xyz, int64_t cells_hint,
+                         hipStream_t s);
+
+struct QueryArgs {
+  const float4 *srec;
+  const int32_t *cell_start;
+  const CloudInfo *clouds;
+  const float4 *qrec;
+  const int4 *qcell;
+  const int32_t *qpad;  // nullable
+  int32_t nq;
+  int32_t k;
+  int32_t R;
+  float cut_r2;
+  int32_t pad_mode;
+  int32_t dist_mode;
+  float pad_dist;
+  int32_t idx64;
+  void *out_idx;
+  float *out_d;
+  int32_t exact;
+  int32_t passes;        // bit 0 reset, bit 1 first pass, bit 2 expansion
+  int32_t *scalars;
+  int32_t *pend_q;       // [nq]
+  uint64_t *pend_keys;   // [nq * k]
+  // batched mode
+  const int32_t *k_per_cloud;
+  const int64_t *out_offsets;
+  int32_t num_clouds;
+  int32_t num_items;     // work items (cloud, 64-query tile)
+  const int32_t *item_cloud;   // unused when 0 items
+};
+
+hipError_t launch_query(const QueryArgs &a, hipStream_t s);
+hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s);
+
+int kclass_of(int k);  // 8, 16, 32 or 64
+hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
+                                  hipStream_t s);
+hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
+                        int32_t B, int32_t *prefix, int32_t *scalars,
+                        hipStream_t s);
+int64_t scan_parts_needed(int64_t n);
+hipError_t launch_scan_host(const int32_t *in, int32_t *out, int64_t n,
+                            int32_t *part, hipStream_t s);
+
+// backward
+int64_t backward_workspace_bytes(int64_t nq, int64_t ns, int32_t k);
+hipError_t launch_dist_backward(const float *q, int64_t nq, const float *p,
+                                int64_t ns, const int64_t *idx, int32_t k,
+                                const float *grad_d, float *grad_q,
+                                float *grad_p, void *ws, hipStream_t s);
+hipError_t launch_idw_forward(const float *feat, int64_t ns, int32_t C,
+                              const int64_t *idx, const float *w, int64_t nq,
+                              int32_t k, float *out, hipStream_t s);
+hipError_t launch_idw_backward(const float *feat, int64_t ns, int32_t C,
+                               const int64_t *idx, const float *w, int64_t nq,
+                               int32_t k, const float *grad_out,
+                               float *grad_feat, float *grad_w, void *ws,
+                               hipStream_t s);
+
+}  // namespace fk

@@ -1,0 +1,677 @@
+// Grid kNN query kernels for gfx950 (wave64).
+//
+// Semantics follow FlashKNN_Query_dynamic_load_kernel
+// (FlashKNN/csrc/flash_knn_query_dynamic_load.cu:469-652): candidates are
+// the support points of the query's cloud in the cells within R of the
+// query's cell (R = 1: 27 cells), d = ((dx*dx + dy*dy) + dz*dz) with
+// dx = p - q (:601-604, evaluated here without contraction), a candidate
+// enters only if d < cut_r2 (:606), slots without a candidate hold
+// (pad, +inf) (:562), rows are written ascending.  Ties are broken by the
+// smaller support id (the reference keeps the first one scanned).
+//
+// Mapping (designed for 64-lane wavefronts, not a translation of the
+// reference's lane groups): one lane owns one query; a wave owns 64
+// consecutive queries of the cell-sorted order, so its lanes walk the same or
+// adjacent column segments and their candidate loads coalesce or broadcast.
+// Each lane keeps its best KC keys sorted in VGPRs as u64 (d bits << 32 | id):
+// for d >= 0 the u64 order is the (d, id) order, so one 64-bit compare
+// decides both the distance and the tie.  Accepted candidates are appended
+// to a per-lane LDS buffer (one ds_write, no divergent insertion); when any
+// lane's buffer is nearly full the wave runs one bitonic sort of the buffer
+// and a bitonic merge into the register list.
+//
+// FK_MODE_EXACT adds a certification per query: if the K-th distance is
+// below the distance from the query to the nearest face of its scanned
+// block (faces beyond which the cloud has no cells do not count), no point
+// outside the block can enter, and the result equals brute force.  The rest
+// go to a pending list and a second kernel grows their block shell by shell
+// (radius from the current K-th distance, doubled while it is infinite)
+// until they certify.
+#include "fk_internal.h"
+
+#include <cfloat>
+#include <climits>
+
+namespace fk {
+
+namespace {
+
+constexpr int kUnroll = 4;
+constexpr uint64_t kMaxKey = ~0ull;
+constexpr uint32_t kInfBits = 0x7f800000u;
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+__device__ __forceinline__ float sqdist(float px, float py, float pz, float qx,
+                                        float qy, float qz) {
+  const float dx = __fsub_rn(px, qx);
+  const float dy = __fsub_rn(py, qy);
+  const float dz = __fsub_rn(pz, qz);
+  return __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)),
+                   __fmul_rn(dz, dz));
+}
+
+__device__ __forceinline__ uint64_t make_key(float d, uint32_t id) {
+  return ((uint64_t)__float_as_uint(d) << 32) | id;
+}
+
+__device__ __forceinline__ void cswap(uint64_t &a, uint64_t &b, bool asc) {
+  const uint64_t x = a, y = b;
+  const bool sw = asc ? (x > y) : (x < y);
+  a = sw ? y : x;
+  b = sw ? x : y;
+}
+
+// Ascending bitonic sort of N (power of two) register keys.
+template <int N>
+__device__ __forceinline__ void bitonic_sort(uint64_t (&a)[N]) {
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int l = i ^ j;
+        if (l > i) cswap(a[i], a[l], (i & k) == 0);
+      }
+    }
+  }
+}
+
+// Bitonic input -> ascending.
+template <int N>
+__device__ __forceinline__ void bitonic_merge(uint64_t (&a)[N]) {
+#pragma unroll
+  for (int j = N >> 1; j > 0; j >>= 1) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int l = i ^ j;
+      if (l > i) cswap(a[i], a[l], true);
+    }
+  }
+}
+
+// Per-lane running top-k (sorted ascending) + LDS append buffer.
+//
+// The k real entries live in the upper k slots of a KC-slot register list;
+// slots [0, KC-k) hold key 0, which sorts below every real key and so stays
+// there through every merge.  The k-th best is then always top[KC-1]: no
+// register is ever indexed by the runtime k (a runtime index would force the
+// whole list into scratch memory).
+template <int KC>
+struct TopK {
+  uint64_t top[KC];
+  uint64_t *lb;     // this lane's buffer: slot s at lb[s * 64]
+  int cnt;
+  int k;
+  int base;         // KC - k: first real slot
+  uint64_t cutkey;
+  uint64_t thr;     // accept key < thr
+
+  __device__ __forceinline__ void init(uint64_t *lane_buf, int k_, float cut_r2,
+                                       uint32_t pad) {
+    lb = lane_buf;
+    cnt = 0;
+    k = k_;
+    base = KC - k_;
+    cutkey = make_key(cut_r2, 0u);
+    const uint64_t padkey = ((uint64_t)kInfBits << 32) | pad;
+#pragma unroll
+    for (int i = 0; i < KC; ++i) top[i] = i < base ? 0ull : padkey;
+    thr = cutkey;  // (inf, pad) >= cutkey for every cut
+  }
+
+  __device__ __forceinline__ void refresh_thr() {
+    thr = top[KC - 1] < cutkey ? top[KC - 1] : cutkey;
+  }
+
+  // Merge the buffered keys into the sorted list (uniform work per lane).
+  __device__ __forceinline__ void flush() {
+    uint64_t b[KC];
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      const uint64_t v = lb[i * kWave];
+      b[i] = i < cnt ? v : kMaxKey;
+    }
+    bitonic_sort<KC>(b);
+#pragma unroll
+    for (int i = 0; i < KC; ++i) {
+      const uint64_t o = b[KC - 1 - i];
+      top[i] = o < top[i] ? o : top[i];
+    }
+    bitonic_merge<KC>(top);
+    cnt = 0;
+    refresh_thr();
+  }
+
+  __device__ __forceinline__ void push(uint64_t key, bool ok) {
+    if (ok && key < thr) {
+      lb[cnt * kWave] = key;
+      ++cnt;
+    }
+  }
+
+  __device__ __forceinline__ uint64_t kth() const { return top[KC - 1]; }
+};
+
+// Scan support records [b, e) for this lane's query.  Wave-uniform trip
+// count (the longest lane); a merge runs whenever any lane could overflow.
+template <int KC>
+__device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, float qx,
+                                           float qy, float qz,
+                                           const float4 *__restrict__ srec) {
+  const int len = e - b;
+  for (int s = 0; wave_any(s < len); s += kUnroll) {
+    if (wave_any(t.cnt > KC - kUnroll)) t.flush();
+    float4 r[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const bool ok = s + u < len;
+      r[u] = srec[ok ? b + s + u : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const float d = sqdist(r[u].x, r[u].y, r[u].z, qx, qy, qz);
+      t.push(make_key(d, __float_as_uint(r[u].w)), s + u < len);
+    }
+  }
+}
+
+// Column segment (cx, cy, z0..z1) of the query's cloud box -> [b, e).
+__device__ __forceinline__ void column_range(const CloudInfo &ci,
+                                             const int32_t *__restrict__ cs,
+                                             int cx, int cy, int z0, int z1,
+                                             int &b, int &e) {
+  b = e = 0;
+  if (cx < 0 || cx >= ci.nx || cy < 0 || cy >= ci.ny) return;
+  z0 = z0 < 0 ? 0 : z0;
+  z1 = z1 >= ci.nz ? ci.nz - 1 : z1;
+  if (z0 > z1) return;
+  const int col = ci.base + (cx * ci.ny + cy) * ci.nz;
+  b = cs[col + z0];
+  e = cs[col + z1 + 1];
+}
+
+// Scan the full block of half-width R around the query's cell.
+template <int KC>
+__device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
+                                           const CloudInfo &ci, int4 qc,
+                                           float4 qr, int R, bool active) {
+  for (int dx = -R; dx <= R; ++dx) {
+    for (int dy = -R; dy <= R; ++dy) {
+      int b = 0, e = 0;
+      if (active)
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - R,
+                     qc.z + R, b, e);
+      scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
+    }
+  }
+}
+
+// Scan the shell Ro < max(|dx|,|dy|,|dz|) <= Rn (divergent; pending queries).
+template <int KC>
+__device__ void scan_shell(TopK<KC> &t, const QueryArgs &a, const CloudInfo &ci,
+                           int4 qc, float4 qr, int Ro, int Rn) {
+  // columns clamped to the box; z bounds in 64 bits, clamped to [-1, nz]
+  // (column_range clips them to the box), so no int overflow for queries
+  // far outside their cloud's box.
+  const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
+  const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
+  auto zc = [&](long long v) {
+    return (int)(v < -1 ? -1 : (v > ci.nz ? ci.nz : v));
+  };
+  const long long zq = qc.z;
+  const int zlo = zc(zq - Rn), zhi = zc(zq + Rn);
+  const int zin_lo = zc(zq - Ro - 1), zin_hi = zc(zq + Ro + 1);
+  for (int dx = x0; dx <= x1; ++dx) {
+    for (int dy = y0; dy <= y1; ++dy) {
+      const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
+      int b, e;
+      if (inner) {
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zin_lo, b, e);
+        scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zin_hi, zhi, b, e);
+      } else {
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zhi, b, e);
+      }
+      scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
+    }
+  }
+}
+
+// Distance (scaled cell units) from the query to the nearest face of its
+// block beyond which the cloud's box still has cells; +inf when the block
+// covers the box.
+__device__ __forceinline__ double cert_gap(const CloudInfo &ci, int4 qc,
+                                           float4 qr, int R, double &umax) {
+  const float inv = ci.inv_cell;
+  const float ux = __fmul_rn(qr.x, inv), uy = __fmul_rn(qr.y, inv),
+              uz = __fmul_rn(qr.z, inv);
+  umax = fmax(fabs((double)ux), fmax(fabs((double)uy), fabs((double)uz)));
+  double gap = HUGE_VAL;
+  const double f[3] = {(double)ux - (double)(qc.x + ci.minx),
+                       (double)uy - (double)(qc.y + ci.miny),
+                       (double)uz - (double)(qc.z + ci.minz)};
+  const int c[3] = {qc.x, qc.y, qc.z};
+  const int n[3] = {ci.nx, ci.ny, ci.nz};
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    if (c[ax] - R > 0) gap = fmin(gap, f[ax] + R);
+    if (c[ax] + R < n[ax] - 1) gap = fmin(gap, (double)(R + 1) - f[ax]);
+  }
+  return gap;
+}
+
+// eff = min(K-th distance, cut): every point with d < eff could still enter.
+__device__ __forceinline__ bool certified(double gap, float eff, float inv,
+                                          double umax, int R) {
+  if (gap == HUGE_VAL) return true;
+  if (!(eff < INFINITY)) return false;
+  const double g = gap - 2.4e-7 * (umax + R + 2.0);
+  if (g <= 0.0) return false;
+  const double s = (double)eff * (double)inv * (double)inv * (1.0 + 2e-6);
+  return s < g * g;
+}
+
+// Smallest block half-width that can certify `eff`, at least Ro + 1.
+__device__ __forceinline__ int next_radius(const CloudInfo &ci, int4 qc,
+                                           float4 qr, int Ro, float eff,
+                                           int Rcover) {
+  int Rn;
+  if (eff < INFINITY) {
+    const float inv = ci.inv_cell;
+    const float u[3] = {__fmul_rn(qr.x, inv), __fmul_rn(qr.y, inv),
+                        __fmul_rn(qr.z, inv)};
+    const int c[3] = {qc.x + ci.minx, qc.y + ci.miny, qc.z + ci.minz};
+    const double rho = sqrt((double)eff * (1.0 + 4e-6)) * (double)inv;
+    double need = (double)(Ro + 1);
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+      const double f = (double)u[ax] - (double)c[ax];
+      const double m = 2.4e-7 * (fabs((double)u[ax]) + Rcover + 2.0) + 1e-9;
+      need = fmax(need, ceil(rho + m - f));
+      need = fmax(need, ceil(rho + m - 1.0 + f));
+    }
+    Rn = need > (double)Rcover ? Rcover : (int)need;
+  } else {
+    const long long r2 = 2ll * Ro + 1;
+    Rn = r2 > Rcover ? Rcover : (int)r2;
+  }
+  if (Rn > Rcover) Rn = Rcover;
+  if (Rn <= Ro) Rn = Ro + 1;
+  return Rn;
+}
+
+__device__ __forceinline__ int cover_radius(const CloudInfo &ci, int4 qc) {
+  int r = max(qc.x, ci.nx - 1 - qc.x);
+  r = max(r, max(qc.y, ci.ny - 1 - qc.y));
+  r = max(r, max(qc.z, ci.nz - 1 - qc.z));
+  return r < 0 ? 0 : r;
+}
+
+__device__ __forceinline__ uint32_t pad_value(const QueryArgs &a, float4 qr,
+                                              int qi) {
+  if (a.pad_mode == FK_PAD_SELF) return __float_as_uint(qr.w);
+  if (a.pad_mode == FK_PAD_MINUS1) return 0xffffffffu;
+  return (uint32_t)a.qpad[qi];
+}
+
+// Output slot of row `row` (batched: per-cloud K and row base).
+__device__ __forceinline__ int64_t out_base(const QueryArgs &a,
+                                            const CloudInfo &ci, int cloud,
+                                            int row, int k) {
+  if (a.k_per_cloud)
+    return a.out_offsets[cloud] + (int64_t)(row - ci.first_id) * k;
+  return (int64_t)row * k;
+}
+
+template <int KC>
+__device__ __forceinline__ void write_row(const QueryArgs &a, const TopK<KC> &t,
+                                          int64_t o) {
+  // real entry j lives in register slot base + j
+  const int64_t ob = o - t.base;
+#pragma unroll
+  for (int j = 0; j < KC; ++j) {
+    if (j >= t.base) {
+      const uint64_t key = t.top[j];
+      const int id = (int)(uint32_t)key;
+      if (a.idx64)
+        ((int64_t *)a.out_idx)[ob + j] = (int64_t)id;
+      else
+        ((int32_t *)a.out_idx)[ob + j] = id;
+      if (a.out_d) {
+        const uint32_t db = (uint32_t)(key >> 32);
+        float d = __uint_as_float(db);
+        if (db == kInfBits)
+          d = a.pad_dist;
+        else if (a.dist_mode == FK_DIST_SQRT)
+          d = sqrtf(d);
+        a.out_d[ob + j] = d;
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
+  const uint64_t m = __ballot(pred);
+  const int lane = threadIdx.x & (kWave - 1);
+  int base = 0;
+  if (m != 0ull) {
+    const int leader = __builtin_ctzll(m);
+    if (lane == leader) base = atomicAdd(counter, __popcll(m));
+    base = __shfl(base, leader, kWave);
+  }
+  const uint64_t below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+  return base + __popcll(below);
+}
+
+// One wave: queries [q0, q0 + 64) of the sorted query list, all lanes of one
+// K class.  Used by the plain kernel and by the persistent batched kernel.
+template <int KC>
+__device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
+                                         uint64_t *lane_buf) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int qi = q0 + lane;
+  const bool active = qi < qend;
+  const int qs = active ? qi : (qend - 1);
+  const float4 qr = a.qrec[qs];
+  const int4 qc = a.qcell[qs];
+  const CloudInfo ci = a.clouds[qc.w];
+  const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
+  TopK<KC> t;
+  t.init(lane_buf, k, a.cut_r2, pad_value(a, qr, qs));
+  scan_block<KC>(t, a, ci, qc, qr, a.R, active);
+  if (wave_any(t.cnt > 0)) t.flush();
+
+  bool done = true;
+  if (a.exact) {
+    double umax;
+    const double gap = cert_gap(ci, qc, qr, a.R, umax);
+    const uint64_t kk = t.kth();
+    float eff = __uint_as_float((uint32_t)(kk >> 32));
+    eff = eff < a.cut_r2 ? eff : a.cut_r2;
+    done = certified(gap, eff, ci.inv_cell, umax, a.R);
+    const int nc = __popcll(__ballot(active && done));
+    if (lane == 0 && nc) atomicAdd(a.scalars + kStatCertified, nc);
+    const int slot = wave_prefix_slot(active && !done, a.scalars + kPending);
+    if (active && !done && slot >= a.nq) {
+      // the pending counter was not reset before this pass: refuse to write
+      raise_status(a.scalars, FK_ERR_STATE);
+    } else if (active && !done) {
+      a.pend_q[slot] = qs;
+      const int64_t pb = (int64_t)slot * a.k - t.base;
+#pragma unroll
+      for (int j = 0; j < KC; ++j)
+        if (j >= t.base) a.pend_keys[pb + j] = t.top[j];
+    }
+  }
+  if (active && done) {
+    const int row = __float_as_int(qr.w);
+    write_row<KC>(a, t, out_base(a, ci, qc.w, row, k));
+  }
+}
+
+template <int KC>
+constexpr int waves_per_block() {
+  return KC <= 16 ? 4 : (KC == 32 ? 2 : 1);
+}
+
+template <int KC>
+__global__ void __launch_bounds__(64 * waves_per_block<KC>())
+k_query(QueryArgs a) {
+  constexpr int WPB = waves_per_block<KC>();
+  __shared__ uint64_t sbuf[WPB * KC * kWave];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  const int blk = xcd_remap(blockIdx.x, gridDim.x);
+  const int q0 = (blk * WPB + w) * kWave;
+  // wave-uniform exits: past the end, or the build/attach raised an error
+  // (its tables may then be incomplete)
+  if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
+  run_tile<KC>(a, q0, a.nq, sbuf + w * KC * kWave + lane);
+}
+
+// Pending queries of FK_MODE_EXACT: grow the block until certified.
+template <int KC>
+__global__ void __launch_bounds__(64 * waves_per_block<KC>())
+k_query_expand(QueryArgs a) {
+  constexpr int WPB = waves_per_block<KC>();
+  __shared__ uint64_t sbuf[WPB * KC * kWave];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.scalars[kPending], a.nq);
+  const int nwaves = gridDim.x * WPB;
+  for (int base = (blockIdx.x * WPB + w) * kWave; base < npend;
+       base += nwaves * kWave) {
+    const int slot = base + lane;
+    const bool active = slot < npend;
+    const int qs = a.pend_q[active ? slot : base];
+    const float4 qr = a.qrec[qs];
+    const int4 qc = a.qcell[qs];
+    const CloudInfo ci = a.clouds[qc.w];
+    const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
+    TopK<KC> t;
+    t.init(sbuf + w * KC * kWave + lane, k, a.cut_r2, pad_value(a, qr, qs));
+    const int64_t pb = (int64_t)(active ? slot : base) * a.k - t.base;
+#pragma unroll
+    for (int j = 0; j < KC; ++j)
+      if (j >= t.base) t.top[j] = a.pend_keys[pb + j];
+    t.refresh_thr();
+    const int Rcover = cover_radius(ci, qc);
+    int Ro = a.R;
+    bool done = !active || Ro >= Rcover;
+    while (wave_any(!done)) {
+      if (!done) {
+        const uint64_t kk = t.kth();
+        float eff = __uint_as_float((uint32_t)(kk >> 32));
+        eff = eff < a.cut_r2 ? eff : a.cut_r2;
+        const int Rn = next_radius(ci, qc, qr, Ro, eff, Rcover);
+        scan_shell<KC>(t, a, ci, qc, qr, Ro, Rn);
+        if (t.cnt > 0) t.flush();
+        Ro = Rn;
+        double umax;
+        const double gap = cert_gap(ci, qc, qr, Ro, umax);
+        const uint64_t k2 = t.kth();
+        float eff2 = __uint_as_float((uint32_t)(k2 >> 32));
+        eff2 = eff2 < a.cut_r2 ? eff2 : a.cut_r2;
+        done = Ro >= Rcover || certified(gap, eff2, ci.inv_cell, umax, Ro);
+      }
+    }
+    if (active) {
+      if (lane == 0) atomicAdd(a.scalars + kStatExpanded, __popcll(__ballot(true)));
+      const int row = __float_as_int(qr.w);
+      write_row<KC>(a, t, out_base(a, ci, qc.w, row, k));
+    }
+  }
+}
+
+// ----------------------------------------------------- batched persistent
+
+// Work item i -> (cloud, first query of the tile) using the per-cloud tile
+// prefix item_cloud[c] (exclusive, int32[num_clouds + 1]).
+__device__ __forceinline__ int item_to_cloud(const int32_t *prefix, int B,
+                                             int item) {
+  int lo = 0, hi = B;  // largest c with prefix[c] <= item
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (prefix[mid] <= item) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+template <int MAXKC>
+__global__ void __launch_bounds__(64 * waves_per_block<MAXKC>())
+k_batched(QueryArgs a) {
+  constexpr int WPB = waves_per_block<MAXKC>();
+  __shared__ uint64_t sbuf[WPB * MAXKC * kWave];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  uint64_t *lane_buf = sbuf + w * MAXKC * kWave + lane;
+  const int32_t *prefix = a.item_cloud;
+  const int nitems = a.scalars[kStatus] != 0 ? 0 : prefix[a.num_clouds];
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(a.scalars + kQueueHead, 1);
+    item = __shfl(item, 0, kWave);
+    if (item >= nitems) break;
+    const int c = item_to_cloud(prefix, a.num_clouds, item);
+    const CloudInfo ci = a.clouds[c];
+    const int qb = a.cell_start[ci.base];
+    const int qe = a.cell_start[ci.base + ci.nx * ci.ny * ci.nz];
+    const int q0 = qb + (item - prefix[c]) * kWave;
+    const int k = a.k_per_cloud[c];
+    if (k <= 8 || MAXKC == 8)
+      run_tile<8>(a, q0, qe, lane_buf);
+    else if (k <= 16 || MAXKC == 16)
+      run_tile<(MAXKC < 16 ? MAXKC : 16)>(a, q0, qe, lane_buf);
+    else if (k <= 32 || MAXKC == 32)
+      run_tile<(MAXKC < 32 ? MAXKC : 32)>(a, q0, qe, lane_buf);
+    else
+      run_tile<MAXKC>(a, q0, qe, lane_buf);
+  }
+}
+
+// Per-cloud tile counts -> exclusive prefix (one block).
+__global__ void __launch_bounds__(1024)
+k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
+        int32_t *prefix, int32_t *scalars) {
+  __shared__ int wsum[16];
+  __shared__ int carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c0 = 0; c0 < B; c0 += 1024) {
+    const int c = c0 + threadIdx.x;
+    int tiles = 0;
+    if (c < B) {
+      const CloudInfo ci = clouds[c];
+      const int n = cell_start[ci.base + ci.nx * ci.ny * ci.nz] - cell_start[ci.base];
+      tiles = (n + kWave - 1) / kWave;
+    }
+    int v = tiles;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(v, o, kWave);
+      if (lane >= o) v += u;
+    }
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    int before = carry;
+    for (int j = 0; j < w; ++j) before += wsum[j];
+    if (c < B) prefix[c] = before + v - tiles;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = before + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    prefix[B] = carry;
+    scalars[kQueueHead] = 0;
+    scalars[kPending] = 0;
+    scalars[kStatCertified] = 0;
+    scalars[kStatExpanded] = 0;
+  }
+}
+
+// Sum over queries of the candidate count |C(q)| of the block of
+// half-width R (the reference's traverse statistic,
+// FlashKNN/functions/FlashKnnWrapper.py:351-352).
+__global__ void k_candidate_count(const QueryArgs a, unsigned long long *total) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long c = 0;
+  if (qi < a.nq && a.scalars[kStatus] == 0) {
+    const int4 qc = a.qcell[qi];
+    const CloudInfo ci = a.clouds[qc.w];
+    for (int dx = -a.R; dx <= a.R; ++dx)
+      for (int dy = -a.R; dy <= a.R; ++dy) {
+        int b, e;
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
+                     qc.z + a.R, b, e);
+        c += (unsigned long long)(e - b);
+      }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(total, c);
+}
+
+__global__ void k_reset_query_scalars(int32_t *scalars) {
+  if (threadIdx.x == 0) {
+    scalars[kPending] = 0;
+    scalars[kStatCertified] = 0;
+    scalars[kStatExpanded] = 0;
+    scalars[kQueueHead] = 0;
+  }
+}
+
+template <int KC>
+hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
+  constexpr int WPB = waves_per_block<KC>();
+  const int64_t waves = ((int64_t)a.nq + kWave - 1) / kWave;
+  const unsigned nb = (unsigned)((waves + WPB - 1) / WPB);
+  if (a.passes & 2)
+    hipLaunchKernelGGL(k_query<KC>, dim3(nb), dim3(64 * WPB), 0, s, a);
+  if (a.exact && (a.passes & 4)) {
+    unsigned ne = nb < 2048 ? nb : 2048;
+    hipLaunchKernelGGL(k_query_expand<KC>, dim3(ne), dim3(64 * WPB), 0, s, a);
+  }
+  return hipGetLastError();
+}
+
+template <int MAXKC>
+hipError_t launch_batched_t(const QueryArgs &a, hipStream_t s) {
+  constexpr int WPB = waves_per_block<MAXKC>();
+  int dev = 0, cus = 256;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  // persistent: a few resident workgroups per CU pull tiles from the queue
+  const int per_cu = MAXKC <= 16 ? 4 : 5;
+  unsigned nb = (unsigned)(cus * per_cu);
+  hipLaunchKernelGGL(k_batched<MAXKC>, dim3(nb), dim3(64 * WPB), 0, s, a);
+  if (a.exact) {
+    hipLaunchKernelGGL(k_query_expand<MAXKC>, dim3(cus * 2), dim3(64 * WPB), 0, s,
+                       a);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int kclass_of(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 : 64)); }
+
+hipError_t launch_query(const QueryArgs &a, hipStream_t s) {
+  if (a.passes & 1)
+    hipLaunchKernelGGL(k_reset_query_scalars, dim3(1), dim3(64), 0, s, a.scalars);
+  if (a.nq <= 0) return hipGetLastError();
+  switch (kclass_of(a.k)) {
+    case 8: return launch_plain<8>(a, s);
+    case 16: return launch_plain<16>(a, s);
+    case 32: return launch_plain<32>(a, s);
+    default: return launch_plain<64>(a, s);
+  }
+}
+
+hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
+                                  hipStream_t s) {
+  if (a.nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_candidate_count, dim3((a.nq + 255) / 256), dim3(256), 0, s,
+                     a, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
+                        int32_t B, int32_t *prefix, int32_t *scalars,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, s, clouds, cell_start, B,
+                     prefix, scalars);
+  return hipGetLastError();
+}
+
+hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s) {
+  switch (kclass_of(max_k)) {
+    case 8: return launch_batched_t<8>(a, s);
+    case 16: return launch_batched_t<16>(a, s);
+    case 32: return launch_batched_t<32>(a, s);
+    default: return launch_batched_t<64>(a, s);
+  }
+}
+
+}  // namespace fk

@@ -1,0 +1,243 @@
+/*
+ * flashknn.h - C ABI of the MI355X (gfx950) FlashKNN grid-kNN library.
+ *
+ * Plain pointers and sizes only (no torch types).  Every pointer named
+ * "device" is HIP device memory; every entry point that takes a `stream`
+ * enqueues its work on that HIP stream (hipStream_t passed as void*, NULL =
+ * the null stream) and returns without a device-wide synchronisation.
+ * Entry points return an fk_status (0 = FK_OK); nothing throws across this
+ * ABI.  fk_last_error() returns a message for the last failure on the
+ * calling thread.
+ *
+ * A grid handle (fk_grid) owns the device tables of one build: cell-sorted
+ * support records, per-cell start offsets of a dense per-cloud cell box,
+ * per-cloud layout, and optionally an attached, cell-sorted query list.
+ * One handle must not be used from two streams at once.
+ *
+ * Reference interfaces replaced (reference = warriorsniu/FlashKNN_ExpRepo):
+ *   fk_morton_encode          FlashKNN_Morton_Encode
+ *                             FlashKNN/csrc/flash_knn_construction.cu:65-103
+ *   fk_morton_neighbor_keys   FlashKNN_Morton_Neighbor_Keys
+ *                             FlashKNN/csrc/flash_knn_construction.cu:105-126
+ *   fk_grid_build_cells       the torch construction in FlashKNN.query
+ *                             (argsort, unique_consecutive, searchsorted,
+ *                             cumsum) FlashKNN/functions/FlashKnnWrapper.py:226-337
+ *   fk_grid_build_xyz         the cell layout for exact queries of
+ *                             pointops.knn_query (no reference construction:
+ *                             the reference scans whole segments,
+ *                             Pointcept/libs/pointops/src/knn_query/
+ *                             knn_query_cuda_kernel.cu:60-104)
+ *   fk_grid_attach_queries    query-side construction of selected_query /
+ *                             back_query (FlashKnnWrapper.py:460-537,686-716)
+ *   fk_query                  FlashKNN_Query_Dynamic_Load (api.cpp:9,
+ *                             flash_knn_query_dynamic_load.cu:781-795),
+ *                             FlashKNN_Selected_Query_DL (:1523-1641),
+ *                             FlashKNN_Back_Query_DL / Nearest_Back_Query_DL
+ *                             (:1373-1521) and pointops knn_query_cuda
+ *                             (Pointcept/libs/pointops/src/knn_query/
+ *                             knn_query_cuda.cpp:7)
+ *   fk_batched_*              no reference counterpart (one persistent launch
+ *                             over many clouds with per-cloud K, hipGraph
+ *                             replay)
+ *   fk_dist_backward          no reference counterpart (FlashKNN.query is
+ *                             @torch.no_grad, FlashKnnWrapper.py:165)
+ *   fk_idw_forward/backward   pointops interpolation_forward/backward_cuda
+ *                             (Pointcept/libs/pointops/src/interpolation/
+ *                             interpolation_cuda_kernel.cu:5-33), with a
+ *                             deterministic segmented backward
+ */
+#ifndef FLASHKNN_H
+#define FLASHKNN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FK_ABI_VERSION 1
+
+typedef enum fk_status {
+  FK_OK = 0,
+  FK_ERR_INVALID_ARG = 1,    /* null pointer, negative size, bad enum     */
+  FK_ERR_K_RANGE = 2,        /* k outside [1, FK_MAX_K]                   */
+  FK_ERR_GRID_TOO_LARGE = 3, /* dense cell box above the cell capacity    */
+  FK_ERR_DOMAIN = 4,         /* batch id / cell coordinate out of range   */
+  FK_ERR_HIP = 5,            /* a HIP runtime call failed                 */
+  FK_ERR_CAPACITY = 6,       /* FK_NO_SYNC build above reserved capacity  */
+  FK_ERR_STATE = 7           /* handle not built / no queries attached    */
+} fk_status;
+
+#define FK_MAX_K 64
+
+/* fk_query_params.mode */
+#define FK_MODE_CELLS 0 /* candidates = support points of the query's cloud
+                           whose cell is within `radius_cells` of the
+                           query's cell on every axis (1 => 27 cells): the
+                           FlashKNN.query candidate set                    */
+#define FK_MODE_EXACT 1 /* exact kNN within the query's cloud: cell scan,
+                           per-query certification, shell expansion of the
+                           uncertified (grids built by fk_grid_build_xyz)  */
+/* fk_query_params.pad_mode: index written to slots that have no neighbour */
+#define FK_PAD_SELF 0      /* the query's own id (FlashKNN)                */
+#define FK_PAD_MINUS1 1    /* -1 (pointops, back_query)                   */
+#define FK_PAD_PER_QUERY 2 /* per-query value given to attach_queries      */
+/* fk_query_params.dist_mode */
+#define FK_DIST_NONE 0 /* out_dist may be NULL                            */
+#define FK_DIST_SQ 1   /* squared fp32 distance                           */
+#define FK_DIST_SQRT 2 /* sqrt of the squared distance (pointops)         */
+/* fk_query_params.query_set */
+#define FK_QUERIES_SELF 0     /* every support point queries the grid      */
+#define FK_QUERIES_ATTACHED 1 /* the list given to fk_grid_attach_queries   */
+/* build flags */
+#define FK_BUILD_NO_SYNC 1 /* never read sizes back to the host: use the
+                              reserved capacity; overflow is reported by
+                              fk_grid_status() (graph-capture safe)         */
+
+typedef struct fk_query_params {
+  int32_t k;            /* neighbours per query, 1..64                     */
+  int32_t mode;         /* FK_MODE_*                                        */
+  int32_t radius_cells; /* FK_MODE_CELLS: block half-width (>= 1)          */
+  float cut_r2;         /* keep d < cut_r2 (strict); INFINITY = no cut      */
+  int32_t idx_64;       /* 1: out_idx is int64, 0: int32                    */
+  int32_t pad_mode;     /* FK_PAD_*                                         */
+  int32_t dist_mode;    /* FK_DIST_*                                        */
+  float pad_dist;       /* value written to out_dist for padded slots      */
+  int32_t query_set;    /* FK_QUERIES_*                                     */
+  int32_t passes;       /* 0 = all; else a mask of the launches fk_query
+                           enqueues: 1 counter reset, 2 first (cell block)
+                           pass, 4 expansion pass (FK_MODE_EXACT).  Lets a
+                           profiler time one kernel alone.                  */
+  int32_t reserved[2];
+} fk_query_params;
+
+typedef struct fk_grid fk_grid;
+
+/* ------------------------------------------------------------- library */
+const char *fk_status_string(int status);
+const char *fk_last_error(void);
+int fk_abi_version(void);
+
+/* ------------------------------------------------ Morton keys (drop-in) */
+/* keys[i] = 48-bit Morton key of grid_coord[i] (16 bits per axis, x at bit
+ * 3i+2, y at 3i+1, z at 3i) | batch_idx[i] << 48.  All pointers device. */
+int fk_morton_encode(const int64_t *grid_coord, const int64_t *batch_idx,
+                     int64_t n, int64_t *keys, void *stream);
+/* keys[27p+j] = key of grid_down[p] + offset j, offsets [0,-1,1]^3 in ij
+ * order (centre first). */
+int fk_morton_neighbor_keys(const int64_t *grid_down,
+                            const int64_t *parent_batch, int64_t num_parents,
+                            int64_t *keys, void *stream);
+
+/* ---------------------------------------------------------------- grids */
+int fk_grid_create(fk_grid **out);
+int fk_grid_destroy(fk_grid *grid);
+/* Pre-size the device tables (needed before FK_BUILD_NO_SYNC builds). */
+int fk_grid_reserve(fk_grid *grid, int64_t max_points, int64_t max_cells,
+                    int32_t max_clouds, int64_t max_queries);
+
+/* Build from integer voxel coordinates (FlashKNN.query layout):
+ * cell = grid_coord >> num_down (int64[n,3], device), cloud = batch_idx[i]
+ * (int64[n], device, NULL => one cloud) in [0, num_clouds).  xyz (float[n,3],
+ * device) gives the coordinates distances are taken on; NULL => grid_coord
+ * converted to float.  Support ids are 0..n-1. */
+int fk_grid_build_cells(fk_grid *grid, const float *xyz,
+                        const int64_t *grid_coord, const int64_t *batch_idx,
+                        int64_t n, int32_t num_clouds, int32_t num_down,
+                        int32_t flags, void *stream);
+
+/* Build from coordinates (exact-query layout): point i belongs to cloud c
+ * with offsets[c-1] <= i < offsets[c] (int32 cumulative ends, device);
+ * cell = floor(xyz * inv_cell[c]) (float[num_clouds], device).  Support ids
+ * are 0..n-1. */
+int fk_grid_build_xyz(fk_grid *grid, const float *xyz, const int32_t *offsets,
+                      int32_t num_clouds, int64_t n, const float *inv_cell,
+                      int32_t flags, void *stream);
+
+/* Attach a query list in the grid's cell space.  For grids built from
+ * cells: qcell (int64[nq,3], device) are voxel coordinates (shifted by the
+ * grid's num_down) and qbatch (int64[nq], device, nullable) the cloud.  For
+ * grids built from xyz: qcell must be NULL; the cell is floor(qxyz*inv) and
+ * the cloud comes from qoffsets (int32 cumulative ends over the queries).
+ * Results of query i are written to output row i.  qpad (int32[nq], device,
+ * nullable) supplies the FK_PAD_PER_QUERY value. */
+int fk_grid_attach_queries(fk_grid *grid, const float *qxyz,
+                           const int64_t *qcell, const int64_t *qbatch,
+                           const int32_t *qoffsets, int64_t nq,
+                           const int32_t *qpad, int32_t flags, void *stream);
+
+/* Device status word of the last build (FK_OK or the first error a kernel
+ * raised); reading it synchronises the stream. */
+int fk_grid_status(fk_grid *grid, void *stream);
+/* Host-side sizes of the last build (points, clouds, cells; -1 if unknown
+ * in FK_BUILD_NO_SYNC mode). */
+int fk_grid_info(fk_grid *grid, int64_t *num_points, int32_t *num_clouds,
+                 int64_t *num_cells, int64_t *num_queries);
+/* Device pointers of the build (for tests and the backward): sorted support
+ * records float4 (x,y,z,id bits), cell start table int32[num_cells+1]. */
+int fk_grid_tables(fk_grid *grid, const void **sorted_records,
+                   const int32_t **cell_start);
+
+/* ---------------------------------------------------------------- query */
+/* Row r of out_idx/out_dist (k entries, ascending by (d, id)) belongs to
+ * support point r (FK_QUERIES_SELF) or attached query r.  out_idx is
+ * int32/int64 per params->idx_64; out_dist may be NULL when dist_mode is
+ * FK_DIST_NONE. */
+int fk_query(fk_grid *grid, const fk_query_params *params, void *out_idx,
+             float *out_dist, void *stream);
+/* Statistics of the last FK_MODE_EXACT query: queries certified by the
+ * first pass, and those resolved by shell expansion.  Synchronises. */
+int fk_query_stats(fk_grid *grid, int64_t *first_pass_certified,
+                   int64_t *expanded, void *stream);
+/* Sum over the query set of the number of candidates in the block of
+ * half-width radius_cells (FlashKNN.query's traverse_info statistic,
+ * FlashKnnWrapper.py:351-352, is this sum / N).  Synchronises. */
+int fk_candidate_count(fk_grid *grid, int32_t radius_cells, int32_t query_set,
+                       int64_t *total, void *stream);
+
+/* -------------------------------------- batched persistent query (C4) */
+/* One persistent launch over all clouds of a grid built by
+ * fk_grid_build_xyz or fk_grid_build_cells, with a per-cloud k.
+ * k_per_cloud: int32[num_clouds] (device).  out_offsets: int64[num_clouds]
+ * (device) start of cloud c's rows in out_idx/out_dist, in elements
+ * (row r of cloud c lives at out_offsets[c] + (r - first point of c) * k_c).
+ * params->k must be the maximum of k_per_cloud. */
+int fk_batched_query(fk_grid *grid, const fk_query_params *params,
+                     const int32_t *k_per_cloud, const int64_t *out_offsets,
+                     void *out_idx, float *out_dist, void *stream);
+/* Capture fk_batched_query into a hipGraph owned by the handle (the
+ * arguments are frozen); fk_graph_replay launches it on `stream`. */
+int fk_batched_capture(fk_grid *grid, const fk_query_params *params,
+                       const int32_t *k_per_cloud, const int64_t *out_offsets,
+                       void *out_idx, float *out_dist, void *stream);
+int fk_graph_replay(fk_grid *grid, void *stream);
+
+/* ------------------------------------------------------------- backward */
+/* Gradient of d_ij = ||q_i - p_idx(i,j)||^2 given grad_d [nq,k]:
+ * grad_q[i] = sum_j 2 g_ij (q_i - p_j)  (float[nq,3], overwritten)
+ * grad_p[m] = -sum_{idx=m} 2 g_ij (q_i - p_m) (float[ns,3], overwritten),
+ * summed in (i, j) order per m (deterministic segmented reduction over a
+ * reverse CSR).  idx is int64 [nq,k]; entries < 0 are skipped.  workspace
+ * must hold fk_backward_workspace_bytes(nq, ns, k) bytes (device). */
+int64_t fk_backward_workspace_bytes(int64_t nq, int64_t ns, int32_t k);
+int fk_dist_backward(const float *q, int64_t nq, const float *p, int64_t ns,
+                     const int64_t *idx, int32_t k, const float *grad_d,
+                     float *grad_q, float *grad_p, void *workspace,
+                     void *stream);
+
+/* IDW: out[i,:] = sum_j w[i,j] * feat[idx[i,j],:]  (w given, [nq,k]). */
+int fk_idw_forward(const float *feat, int64_t ns, int32_t channels,
+                   const int64_t *idx, const float *w, int64_t nq, int32_t k,
+                   float *out, void *stream);
+/* grad_feat[m,:] = sum_{idx=m} w_ij grad_out[i,:] (segmented, deterministic);
+ * grad_w[i,j] = <grad_out[i,:], feat[idx_ij,:]>.  Either output may be NULL.
+ * workspace as for fk_dist_backward. */
+int fk_idw_backward(const float *feat, int64_t ns, int32_t channels,
+                    const int64_t *idx, const float *w, int64_t nq, int32_t k,
+                    const float *grad_out, float *grad_feat, float *grad_w,
+                    void *workspace, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FLASHKNN_H */

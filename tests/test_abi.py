@@ -1,0 +1,70 @@
+"""The C-ABI library loads and exports every entry point include/flashknn.h
+declares (CPU only: no compute calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    src = open(os.path.join(ROOT, "include", "flashknn.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(fk_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    assert "fk_query" in names and "fk_grid_build_cells" in names
+    assert len(names) >= 20
+
+
+def test_library_exports_every_declared_symbol():
+    from flashknn_exprepo_amd import _lib
+    L = _lib.load()
+    missing = [n for n in _declared() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_signatures_cover_header():
+    from flashknn_exprepo_amd import _lib
+    typed = {n for n, _, _ in _lib.SIGNATURES}
+    assert set(_declared()) <= typed
+
+
+def test_query_params_layout_matches_header():
+    from flashknn_exprepo_amd import _lib
+    # 9 int/float fields + passes + 2 reserved, 4 bytes each
+    assert ctypes.sizeof(_lib.QueryParams) == 12 * 4
+
+
+def test_host_only_calls():
+    from flashknn_exprepo_amd import _lib
+    L = _lib.load()
+    assert L.fk_abi_version() == 1
+    assert L.fk_status_string(2) == b"k out of range"
+    h = ctypes.c_void_p()
+    assert L.fk_grid_create(ctypes.byref(h)) == 0 and h.value
+    assert L.fk_grid_destroy(h) == 0
+    assert L.fk_grid_create(None) == _lib.FK_ERR_INVALID_ARG
+    assert b"NULL" in L.fk_last_error()
+    assert L.fk_backward_workspace_bytes(10, 10, 4) > 0
+    assert L.fk_backward_workspace_bytes(-1, 10, 4) == -1
+
+
+def test_status_to_python_errors():
+    from flashknn_exprepo_amd import _lib
+    with pytest.raises(ValueError):
+        _lib.check(_lib.FK_ERR_K_RANGE, "x")
+    with pytest.raises(_lib.FlashKNNError) as e:
+        _lib.check(_lib.FK_ERR_HIP, "x")
+    assert e.value.status == _lib.FK_ERR_HIP
+
+
+def test_no_cpu_fallback():
+    import torch
+    import flashknn_exprepo_amd as fk
+    with pytest.raises(ValueError, match="HIP device"):
+        fk.knn(torch.zeros(4, 3), torch.tensor([4], dtype=torch.int32), 2)

@@ -1,0 +1,435 @@
+"""Parity of the HIP path against the CPU oracle (needs an MI355X).
+
+Bar: neighbour indices and fp32 squared distances bit-exact; sets compared
+row by row where the reference's tie rule differs (the build breaks exact
+ties by the smaller index, which the oracle does too, so rows are compared
+exactly).
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+fk = pytest.importorskip("flashknn_exprepo_amd")
+
+
+def _dev():
+    return torch.device("cuda", 0)
+
+
+def _t(a, dtype=None):
+    t = torch.from_numpy(np.ascontiguousarray(a)).to(_dev())
+    return t if dtype is None else t.to(dtype)
+
+
+def _same(gpu_idx, gpu_d, ref_idx, ref_d):
+    gi = gpu_idx.cpu().numpy().astype(np.int64)
+    gd = gpu_d.cpu().numpy().astype(np.float32)
+    bad = np.where((gi != ref_idx).any(1) | (gd.view(np.uint32) != ref_d.view(np.uint32)).any(1))[0]
+    assert len(bad) == 0, (
+        f"{len(bad)} rows differ; first row {bad[0]}: gpu {gi[bad[0]]} {gd[bad[0]]} "
+        f"ref {ref_idx[bad[0]]} {ref_d[bad[0]]}")
+
+
+# ------------------------------------------------ FlashKNN.query (27 cells)
+
+@pytest.mark.parametrize("k", [1, 2, 8, 16, 24, 32, 48, 64])
+def test_query_uniform_cells(oracle, k):
+    xyz = np.random.default_rng(47).random((8192, 3), dtype=np.float32)
+    grid = np.floor(xyz / np.float32(0.05)).astype(np.int64)
+    ref_i, ref_d = oracle.flashknn_query(grid, None, xyz, k, 0)
+    knn = fk.FlashKNN(num_nbr=k, num_down=0)
+    idx = knn.query(_t(grid), None, _t(xyz))
+    assert idx.dtype == torch.int64 and idx.shape == (8192, k)
+    _same(idx, knn.last_distances, ref_i, ref_d)
+
+
+@pytest.mark.parametrize("num_down", [0, 1, 2])
+def test_query_room_num_down(oracle, num_down):
+    from flashknn_exprepo_amd import synthetic
+    xyz = synthetic.room(seed=48, target=20000)
+    grid = np.floor(xyz / np.float32(0.04)).astype(np.int64)
+    ref_i, ref_d = oracle.flashknn_query(grid, None, xyz, 16, num_down)
+    knn = fk.FlashKNN(num_nbr=16, num_down=num_down)
+    idx = knn.query(_t(grid), None, _t(xyz))
+    _same(idx, knn.last_distances, ref_i, ref_d)
+
+
+def test_query_batches_and_negative_cells(oracle):
+    rng = np.random.default_rng(3)
+    xyz = (rng.random((6000, 3), dtype=np.float32) * 4 - 2).astype(np.float32)
+    grid = np.floor(xyz / np.float32(0.1)).astype(np.int64)
+    batch = rng.integers(0, 5, 6000)
+    ref_i, ref_d = oracle.flashknn_query(grid, batch, xyz, 12, 1)
+    knn = fk.FlashKNN(num_nbr=12, num_down=1)
+    idx = knn.query(_t(grid), _t(batch), _t(xyz))
+    _same(idx, knn.last_distances, ref_i, ref_d)
+
+
+def test_query_duplicates_ties_and_cut(oracle):
+    rng = np.random.default_rng(4)
+    base = np.round(rng.random((500, 3)) * 8).astype(np.float32) / 8
+    xyz = np.concatenate([base, base[:200], base[:50]]).astype(np.float32)
+    grid = np.floor(xyz / np.float32(0.125)).astype(np.int64)
+    for cut in (math.inf, 0.2):
+        ref_i, ref_d = oracle.flashknn_query(grid, None, xyz, 16, 1, cut)
+        knn = fk.FlashKNN(num_nbr=16, num_down=1)
+        idx = knn.query(_t(grid), None, _t(xyz), cut_radius=cut)
+        _same(idx, knn.last_distances, ref_i, ref_d)
+
+
+def test_query_xyz_none_uses_grid(oracle):
+    rng = np.random.default_rng(5)
+    grid = rng.integers(0, 40, (3000, 3))
+    ref_i, ref_d = oracle.flashknn_query(grid, None, None, 8, 2)
+    knn = fk.FlashKNN(num_nbr=8, num_down=2)
+    idx = knn.query(_t(grid), None, None)
+    _same(idx, knn.last_distances, ref_i, ref_d)
+
+
+def test_query_tiny_and_empty(oracle):
+    xyz = np.array([[0.1, 0.1, 0.1]], np.float32)
+    grid = np.zeros((1, 3), np.int64)
+    knn = fk.FlashKNN(num_nbr=4, num_down=0)
+    idx = knn.query(_t(grid), None, _t(xyz))
+    assert idx.cpu().tolist() == [[0, 0, 0, 0]]
+    d = knn.last_distances.cpu().numpy()
+    assert d[0, 0] == 0 and np.isinf(d[0, 1:]).all()
+    idx0 = knn.query(_t(np.zeros((0, 3), np.int64)), None, _t(np.zeros((0, 3), np.float32)))
+    assert idx0.shape == (0, 4)
+
+
+def test_query_mode_validation():
+    knn = fk.FlashKNN(num_nbr=4, num_down=0)
+    g = _t(np.zeros((4, 3), np.int64))
+    with pytest.raises(ValueError):
+        knn.query(g, None, None, candidate_mode="bogus")
+    with pytest.raises(ValueError):
+        knn.query(g, None, None, thread_group_size=12)
+    with pytest.raises(ValueError):
+        fk.FlashKNN(num_nbr=65, num_down=0).query(g, None, None)
+    with pytest.raises(ValueError):
+        knn.query(g, _t(np.array([0, 1, -1, 0])), None)
+
+
+def test_query_modes_identical(oracle):
+    xyz = np.random.default_rng(6).random((4000, 3), dtype=np.float32)
+    grid = np.floor(xyz / np.float32(0.04)).astype(np.int64)
+    knn = fk.FlashKNN(num_nbr=16, num_down=1)
+    a = knn.query(_t(grid), None, _t(xyz))
+    for kw in ({"memory_mode": "GM", "sorting_mode": "SS"}, {"thread_group_size": 8},
+               {"candidate_mode": "shared"}, {"enable_skip": False}):
+        b = knn.query(_t(grid), None, _t(xyz), **kw)
+        assert torch.equal(a, b)
+
+
+def test_traverse_info(oracle):
+    xyz = np.random.default_rng(7).random((3000, 3), dtype=np.float32)
+    grid = np.floor(xyz / np.float32(0.1)).astype(np.int64)
+    info = [None]
+    fk.FlashKNN(num_nbr=8, num_down=0).query(_t(grid), None, _t(xyz), traverse_info=info)
+    # brute count of the 27-cell candidate sets
+    from collections import Counter
+    cnt = Counter(map(tuple, grid))
+    tot = 0
+    for g in grid:
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    tot += cnt.get((g[0] + dx, g[1] + dy, g[2] + dz), 0)
+    assert abs(info[0] - tot / len(grid)) < 1e-9
+
+
+# ------------------------------------------- selected_query / back_query
+
+def test_selected_query(oracle):
+    rng = np.random.default_rng(8)
+    xyz = rng.random((5000, 3), dtype=np.float32)
+    grid = np.floor(xyz / np.float32(0.05)).astype(np.int64)
+    qi = rng.choice(5000, 700, replace=False)
+    ref_i, ref_d = oracle.selected_query(xyz, grid, qi, None, 16, 1)
+    knn = fk.FlashKNN(num_nbr=16, num_down=1)
+    idx = knn.selected_query(_t(xyz), _t(grid), _t(qi), None)
+    _same(idx, knn.last_distances, ref_i, ref_d)
+
+
+@pytest.mark.parametrize("k", [1, 4])
+def test_back_query(oracle, k):
+    from flashknn_exprepo_amd import synthetic
+    xyz = synthetic.room(seed=48, target=20000)
+    sel = synthetic.voxel_first(xyz, 0.08)
+    ref_i, ref_d = oracle.back_query(xyz, sel, 0.16, None, k)
+    out = fk.FlashKNN(num_nbr=k).back_query(_t(xyz), _t(sel), 0.16, 0.08, None)
+    if k == 1:
+        assert torch.equal(out.cpu(), torch.from_numpy(ref_i[:, 0]))
+    else:
+        _same(out[0], out[1], ref_i, ref_d)
+
+
+# ------------------------------------------------------- exact kNN
+
+@pytest.mark.parametrize("k", [1, 8, 16, 32, 64])
+def test_exact_uniform_vs_brute(oracle, k):
+    xyz = np.random.default_rng(47).random((8192, 3), dtype=np.float32)
+    ref_i, ref_d = oracle.brute_knn(xyz, [8192], xyz, [8192], k)
+    idx, d = fk.knn(_t(xyz), _t(np.array([8192], np.int32)), k)
+    _same(idx, d, ref_i, ref_d)
+
+
+@pytest.mark.parametrize("scale", [0.3, 0.6, 1.5])
+def test_exact_room_lidar_any_cell(oracle, scale):
+    from flashknn_exprepo_amd import synthetic
+    rng = np.random.default_rng(9)
+    a = synthetic.subsample_to(synthetic.room(seed=48, target=30000), 12000, rng)
+    b = synthetic.subsample_to(synthetic.lidar(seed=49), 15000, rng)
+    xyz, off = synthetic.pack([a, b])
+    ref_i, ref_d = oracle.brute_knn(xyz, off, xyz, off, 16)
+    eng = fk.ExactKNN()
+    idx, d = eng(_t(xyz), _t(off), 16, scale=scale, stats=True)
+    _same(idx, d, ref_i, ref_d)
+    st = eng.last_stats
+    assert st["certified_first_pass"] + st["expanded"] == len(xyz)
+
+
+def test_exact_other_queries_and_padding(oracle):
+    rng = np.random.default_rng(10)
+    s = rng.random((3000, 3), dtype=np.float32)
+    q = (rng.random((900, 3), dtype=np.float32) * 1.4 - 0.2).astype(np.float32)
+    soff = np.array([5, 1500, 3000], np.int32)      # a 5-point cloud: padding
+    qoff = np.array([300, 600, 900], np.int32)
+    ref_i, ref_d = oracle.brute_knn(s, soff, q, qoff, 8)
+    idx, d = fk.knn(_t(s), _t(soff), 8, _t(q), _t(qoff))
+    _same(idx, d, ref_i, ref_d)
+    assert (idx[:300, 5:] == -1).all()
+
+
+def test_pointops_knn_query(oracle):
+    rng = np.random.default_rng(11)
+    s = rng.random((4000, 3), dtype=np.float32)
+    off = np.array([1000, 4000], np.int32)
+    ref_i, ref_d = oracle.brute_knn(s, off, s, off, 16)
+    idx, dist = fk.pointops.knn_query(16, _t(s), _t(off))
+    assert idx.dtype == torch.int32 and dist.dtype == torch.float32
+    assert np.array_equal(idx.cpu().numpy(), ref_i)
+    np.testing.assert_array_equal(dist.cpu().numpy(), np.sqrt(ref_d))
+
+
+def test_exact_cut_radius(oracle):
+    rng = np.random.default_rng(12)
+    s = rng.random((5000, 3), dtype=np.float32)
+    ref_i, ref_d = oracle.brute_knn(s, [5000], s, [5000], 16, cut_r2=0.003)
+    idx, d = fk.knn(_t(s), _t(np.array([5000], np.int32)), 16, cut_r2=0.003)
+    _same(idx, d, ref_i, ref_d)
+
+
+def test_exact_full_size_room_properties(oracle):
+    """100k room, K=16: exact vs brute force on a 3000-query sample and
+    size-independent properties on every row."""
+    from flashknn_exprepo_amd import synthetic
+    xyz = synthetic.room(seed=48)
+    n = len(xyz)
+    idx, d = fk.knn(_t(xyz), _t(np.array([n], np.int32)), 16)
+    i_np, d_np = idx.cpu().numpy(), d.cpu().numpy()
+    assert (i_np[:, 0] == np.arange(n)).all() and (d_np[:, 0] == 0).all()
+    assert (np.diff(d_np, axis=1) >= 0).all()
+    sample = np.random.default_rng(13).choice(n, 3000, replace=False)
+    ref_i, ref_d = oracle.brute_knn(xyz, [n], xyz[sample], [3000], 16)
+    assert np.array_equal(i_np[sample], ref_i)
+    assert np.array_equal(d_np[sample].view(np.uint32), ref_d.view(np.uint32))
+
+
+# ------------------------------------------------------------- batched
+
+def _batch_case(seed=14):
+    from flashknn_exprepo_amd import synthetic
+    rng = np.random.default_rng(seed)
+    room = synthetic.room(seed=48, target=30000)
+    lid = synthetic.lidar(seed=49)
+    sizes = [2000, 9000, 500, 30000, 4000, 17000]
+    clouds = [synthetic.subsample_to(room if i % 2 == 0 else lid, n, rng)
+              for i, n in enumerate(sizes)]
+    ks = [8, 16, 32, 16, 8, 32]
+    return clouds, ks
+
+
+@pytest.mark.parametrize("exact", [True, False])
+def test_batched_equals_per_cloud(oracle, exact):
+    from flashknn_exprepo_amd import synthetic
+    clouds, ks = _batch_case()
+    xyz, off = synthetic.pack(clouds)
+    bk = fk.BatchedKNN(_t(xyz), _t(off), ks, exact=exact)
+    bk.run()
+    start = 0
+    for c, (cl, k) in enumerate(zip(clouds, ks)):
+        bi, bd = bk.per_cloud(c)
+        if exact:
+            ref_i, ref_d = oracle.brute_knn(cl, [len(cl)], cl, [len(cl)], k)
+        else:
+            # cells are floor(x * inv) with the build's fp32 product
+            inv = np.float32(1.0 / np.float64(np.float32(bk.cell[c].item())))
+            g = np.floor((cl * inv).astype(np.float32)).astype(np.int64)
+            ref_i, ref_d = oracle.grid_knn(cl, g, None, cl, g, None, np.arange(len(cl)), k)
+        _same(bi - start, bd, ref_i, ref_d)
+        start += len(cl)
+
+
+def test_batched_graph_replay_matches_eager():
+    from flashknn_exprepo_amd import synthetic
+    clouds, ks = _batch_case(15)
+    xyz, off = synthetic.pack(clouds)
+    bk = fk.BatchedKNN(_t(xyz), _t(off), ks, exact=True)
+    ei, ed = [t.clone() for t in bk.run()]
+    bk.capture()
+    for rep in range(4):
+        bk.out_idx.fill_(-7)
+        bk.out_dist.fill_(-7.0)
+        torch.cuda.synchronize()
+        ri, rd = bk.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ri, ei) and torch.equal(rd, ed), f"replay {rep}"
+        _ = (torch.randn(1000, device=_dev()) * 2).sum().item()  # eager work between
+
+
+# ------------------------------------------------------------ backward
+
+def test_dist_backward_vs_oracle(oracle):
+    rng = np.random.default_rng(16)
+    p = rng.random((2000, 3), dtype=np.float32)
+    q = rng.random((700, 3), dtype=np.float32)
+    idx, _ = oracle.brute_knn(p, [2000], q, [700], 16)
+    idx[3, 5] = -1
+    g = rng.standard_normal((700, 16)).astype(np.float32)
+    qt = _t(q).requires_grad_(True)
+    pt = _t(p).requires_grad_(True)
+    d = fk.gathered_sqdist(qt, pt, _t(idx))
+    d = torch.where(_t(idx) >= 0, d, torch.zeros_like(d))
+    (d * _t(g)).sum().backward()
+    gq, gp = oracle.dist_backward(q, p, idx, g)
+    for got, ref in ((qt.grad, gq), (pt.grad, gp)):
+        got = got.double().cpu().numpy()
+        rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert rel <= 1e-5, rel
+
+
+def test_backward_deterministic():
+    rng = np.random.default_rng(17)
+    p = _t(rng.random((3000, 3), dtype=np.float32))
+    q = _t(rng.random((3000, 3), dtype=np.float32))
+    idx, _ = fk.knn(p, _t(np.array([3000], np.int32)), 16, q,
+                    _t(np.array([3000], np.int32)))
+    feat = _t(rng.standard_normal((3000, 64)).astype(np.float32))
+    outs = []
+    for _ in range(3):
+        pp, qq, ff = (t.clone().requires_grad_(True) for t in (p, q, feat))
+        out = fk.idw_interpolate(ff, qq, pp, idx)
+        out.square().sum().backward()
+        outs.append([t.grad.clone() for t in (pp, qq, ff)])
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert torch.equal(a, b)
+
+
+def test_idw_fp64_finite_differences(oracle):
+    """C5 check on a 2k-point copy: gradients of the fp32 path vs FP64
+    central differences of the same loss, relative error <= 1e-5."""
+    from flashknn_exprepo_amd import synthetic
+    fine = synthetic.room(seed=51, target=20000)
+    fine = synthetic.subsample_to(fine, 2000, np.random.default_rng(0))
+    coarse = fine[synthetic.voxel_first(fine, 0.16)]
+    idx, _ = oracle.brute_knn(coarse, [len(coarse)], fine, [len(fine)], 16)
+    rng = np.random.default_rng(18)
+    feat = rng.standard_normal((len(coarse), 8))
+    wout = rng.standard_normal((len(fine), 8))
+    eps = 1e-4
+
+    def loss64(qx, px, f):
+        d = ((qx[:, None, :] - px[idx]) ** 2).sum(-1)
+        w = 1.0 / (d + eps)
+        wn = w / w.sum(1, keepdims=True)
+        return ((wn[..., None] * f[idx]).sum(1) * wout).sum()
+
+    qt = _t(fine).requires_grad_(True)
+    pt = _t(coarse).requires_grad_(True)
+    ft = _t(feat.astype(np.float32)).requires_grad_(True)
+    out = fk.idw_interpolate(ft, qt, pt, _t(idx), eps=eps)
+    (out * _t(wout.astype(np.float32))).sum().backward()
+    q64, p64 = fine.astype(np.float64), coarse.astype(np.float64)
+    # analytic fp64 gradient through torch autograd on the CPU (independent)
+    q_ = torch.from_numpy(q64).requires_grad_(True)
+    p_ = torch.from_numpy(p64).requires_grad_(True)
+    f_ = torch.from_numpy(feat).requires_grad_(True)
+    it = torch.from_numpy(idx)
+    d_ = ((q_[:, None, :] - p_[it]) ** 2).sum(-1)
+    w_ = 1.0 / (d_ + eps)
+    wn_ = w_ / w_.sum(1, keepdim=True)
+    ((wn_[..., None] * f_[it]).sum(1) * torch.from_numpy(wout)).sum().backward()
+    for got, ref in ((qt.grad, q_.grad), (pt.grad, p_.grad), (ft.grad, f_.grad)):
+        got = got.double().cpu().numpy()
+        ref = ref.numpy()
+        rel = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        assert rel <= 1e-5, rel
+    # FP64 central differences on a few coordinates pin the fp64 autograd
+    h = 1e-6
+    for (arr, grad) in ((q64, q_.grad.numpy()), (p64, p_.grad.numpy())):
+        for i in range(0, len(arr), max(1, len(arr) // 5)):
+            for a in range(3):
+                old = arr[i, a]
+                arr[i, a] = old + h
+                lp = loss64(q64, p64, feat)
+                arr[i, a] = old - h
+                lm = loss64(q64, p64, feat)
+                arr[i, a] = old
+                num = (lp - lm) / (2 * h)
+                assert abs(num - grad[i, a]) <= 1e-4 * max(1.0, abs(num)), (i, a, num, grad[i, a])
+
+
+def test_pointops_interpolation2_matches_torch():
+    rng = np.random.default_rng(19)
+    s = _t(rng.random((2000, 3), dtype=np.float32))
+    q = _t(rng.random((3000, 3), dtype=np.float32))
+    so, qo = _t(np.array([2000], np.int32)), _t(np.array([3000], np.int32))
+    feat = _t(rng.standard_normal((2000, 32)).astype(np.float32)).requires_grad_(True)
+    out = fk.pointops.interpolation2(s, q, feat, so, qo, 3)
+    ref = fk.pointops.interpolation(s, q, feat.detach(), so, qo, 3)
+    torch.testing.assert_close(out, ref, rtol=1e-6, atol=1e-6)
+    g = torch.randn_like(out)
+    out.backward(g)
+    idx, dist = fk.pointops.knn_query(3, s, so, q, qo)
+    w = 1.0 / (dist.double() + 1e-8)
+    w = w / w.sum(1, keepdim=True)
+    ref_g = torch.zeros(2000, 32, dtype=torch.float64, device=_dev())
+    ref_g.index_add_(0, idx.long().reshape(-1),
+                     (w[..., None] * g.double()[:, None, :]).reshape(-1, 32))
+    torch.testing.assert_close(feat.grad.double(), ref_g, rtol=1e-5, atol=1e-6)
+
+
+# --------------------------------------------------------------- misc
+
+def test_morton_kernel_matches_oracle(oracle):
+    rng = np.random.default_rng(20)
+    g = rng.integers(-5, 1 << 16, (5000, 3))
+    b = rng.integers(0, 100, 5000)
+    k = fk.xyz2key(_t(g[:, 0]), _t(g[:, 1]), _t(g[:, 2]), _t(b))
+    assert np.array_equal(k.cpu().numpy(), oracle.morton_encode(g, b))
+    gd = rng.integers(0, 1000, (300, 3))
+    nb = fk.functions.neighbor_keys(_t(gd), _t(b[:300]))
+    assert np.array_equal(nb.cpu().numpy(), oracle.morton_neighbor_keys(gd, b[:300]))
+
+
+def test_grid_tables_partition(oracle):
+    xyz = np.random.default_rng(21).random((7000, 3), dtype=np.float32)
+    g = fk.Grid()
+    g.build_xyz(_t(xyz), _t(np.array([7000], np.int32)), _t(np.array([10.0], np.float32)))
+    rec, cs = g.tables()
+    rec, cs = rec.cpu().numpy(), cs.cpu().numpy()
+    ids = rec[:, 3].view(np.int32)
+    assert np.array_equal(np.sort(ids), np.arange(7000))
+    assert (np.diff(cs) >= 0).all() and cs[-1] == 7000
+    cells = np.floor(xyz * np.float32(10.0)).astype(np.int64)
+    lin = (cells[:, 0] * 10 + cells[:, 1]) * 10 + cells[:, 2]
+    counts = np.bincount(lin, minlength=1000)
+    assert np.array_equal(np.diff(cs), counts)
+    assert np.array_equal(np.sort(lin[ids]), lin[ids])  # records are cell-sorted

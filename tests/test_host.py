@@ -1,0 +1,84 @@
+"""Host-side logic (CPU only): synthetic workloads, the torch helpers of the
+drop-in surface, cell-size heuristics and layout conversions."""
+import numpy as np
+import pytest
+import torch
+
+
+def test_xyz2key_cpu_matches_oracle(oracle):
+    from flashknn_exprepo_amd.functions import xyz2key
+    rng = np.random.default_rng(0)
+    g = rng.integers(-3, 1 << 16, (400, 3))
+    b = rng.integers(0, 1000, 400)
+    k = xyz2key(torch.from_numpy(g[:, 0]), torch.from_numpy(g[:, 1]),
+                torch.from_numpy(g[:, 2]), torch.from_numpy(b))
+    assert np.array_equal(k.numpy(), oracle.morton_encode(g, b))
+    k8 = xyz2key(torch.tensor([255 + 256]), torch.tensor([0]), torch.tensor([0]), depth=8)
+    assert k8.item() == oracle.morton_key_bitloop(255, 0, 0)
+
+
+def test_cal_recall_multiset():
+    from flashknn_exprepo_amd.functions import cal_recall
+    gt = torch.tensor([[1, 2, 3, 4], [5, 5, 6, 7], [1, 2, 3, 4]])
+    pr = torch.tensor([[4, 3, 2, 1], [5, 6, 6, 9], [9, 9, 9, 1]])
+    r = cal_recall(pr, gt)
+    assert torch.allclose(r, torch.tensor([1.0, 0.5, 0.25]))
+
+
+def test_vanilla_knn_vs_oracle(oracle):
+    from flashknn_exprepo_amd.functions import vanilla_Knn
+    rng = np.random.default_rng(1)
+    s = rng.random((3000, 3), dtype=np.float32)
+    q = rng.random((500, 3), dtype=np.float32)
+    d, i = vanilla_Knn(torch.from_numpy(q), torch.from_numpy(s), 8, 128, 1000)
+    ri, rd = oracle.brute_knn(s, [3000], q, [500], 8, f64=True)
+    same = (np.sort(i.numpy(), 1) == np.sort(ri, 1)).all(1).mean()
+    assert same > 0.99  # matmul expansion rounding may swap near ties
+    np.testing.assert_allclose(d.numpy(), np.sqrt(rd), rtol=1e-3, atol=1e-4)
+
+
+def test_offsets_roundtrip():
+    from flashknn_exprepo_amd import pointops
+    off = torch.tensor([3, 3, 7, 12], dtype=torch.int32)
+    b = pointops.offset2batch(off)
+    assert b.tolist() == [0, 0, 0, 2, 2, 2, 2, 3, 3, 3, 3, 3]
+    # batch2offset drops nothing for contiguous non-empty batches
+    assert pointops.batch2offset(torch.tensor([0, 0, 1, 1, 1])).tolist() == [2, 5]
+
+
+def test_cloud_ids_and_cell_sizes():
+    from flashknn_exprepo_amd.knn import cell_sizes, cloud_ids
+    off = torch.tensor([2, 5, 5, 9], dtype=torch.int32)
+    assert cloud_ids(off, 9).tolist() == [0, 0, 1, 1, 1, 3, 3, 3, 3]
+    rng = np.random.default_rng(2)
+    xyz = torch.from_numpy(np.concatenate([
+        rng.random((1000, 3), dtype=np.float32),
+        rng.random((8000, 3), dtype=np.float32) * 2]))
+    c = cell_sizes(xyz, torch.tensor([1000, 9000], dtype=torch.int32), 16, scale=1.0)
+    # (k V / n)^(1/3): cloud 0: (16/1000)^(1/3); cloud 1: (16*8/8000)^(1/3)
+    np.testing.assert_allclose(c.numpy(), [(16 / 1000) ** (1 / 3), (128 / 8000) ** (1 / 3)],
+                               rtol=0.05)
+
+
+def test_synthetic_uniform_and_voxel_first():
+    from flashknn_exprepo_amd import synthetic
+    u = synthetic.uniform(8192, 47)
+    assert u.shape == (8192, 3) and u.dtype == np.float32
+    assert np.array_equal(u, np.random.default_rng(47).random((8192, 3), dtype=np.float32))
+    pts = np.array([[0.01, 0, 0], [0.02, 0, 0], [0.5, 0, 0], [0.03, 0, 0]], np.float32)
+    assert synthetic.voxel_first(pts, 0.1).tolist() == [0, 2]
+
+
+@pytest.mark.slow
+def test_synthetic_room_size():
+    from flashknn_exprepo_amd import synthetic
+    r = synthetic.room(seed=48, target=20000)
+    assert abs(len(r) - 20000) <= 0.05 * 20000
+    g = np.floor(r / 0.04).astype(np.int64)
+    assert len(np.unique(g, axis=0)) == len(r)  # one point per 4 cm voxel
+
+
+def test_pack_offsets():
+    from flashknn_exprepo_amd import synthetic
+    xyz, off = synthetic.pack([np.zeros((3, 3)), np.ones((5, 3))])
+    assert off.tolist() == [3, 8] and xyz.shape == (8, 3)
