@@ -498,12 +498,14 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
     return fail(FK_ERR_INVALID_ARG, "FK_PAD_PER_QUERY needs attached queries");
   if (p->pad_mode == FK_PAD_SELF && p->query_set != FK_QUERIES_SELF)
     return fail(FK_ERR_INVALID_ARG, "FK_PAD_SELF is for self queries");
-  if (!out_idx) return fail(FK_ERR_INVALID_ARG, "fk_query: out_idx is NULL");
-  if (p->dist_mode != FK_DIST_NONE && !out_dist)
+  const bool self = p->query_set == FK_QUERIES_SELF;
+  const int64_t nq = self ? g->n_points : g->n_queries;
+  // an empty query set may pass NULL outputs (a 0-element tensor has none)
+  if (!out_idx && nq > 0) return fail(FK_ERR_INVALID_ARG, "fk_query: out_idx is NULL");
+  if (p->dist_mode != FK_DIST_NONE && !out_dist && nq > 0)
     return fail(FK_ERR_INVALID_ARG, "fk_query: out_dist is NULL");
   if (std::isnan(p->cut_r2))
     return fail(FK_ERR_INVALID_ARG, "fk_query: cut_r2 is NaN");
-  const bool self = p->query_set == FK_QUERIES_SELF;
   memset(&a, 0, sizeof(a));
   a.srec = g->srec;
   a.cell_start = g->cell_start;

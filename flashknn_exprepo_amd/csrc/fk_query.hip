@@ -475,8 +475,11 @@ k_query_expand(QueryArgs a) {
         done = Ro >= Rcover || certified(gap, eff2, ci.inv_cell, umax, Ro);
       }
     }
+    // count outside any lane-dependent branch: a ballot taken inside
+    // `if (lane == 0)` would see lane 0 alone
+    const int nexp = __popcll(__ballot(active));
+    if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
     if (active) {
-      if (lane == 0) atomicAdd(a.scalars + kStatExpanded, __popcll(__ballot(true)));
       const int row = __float_as_int(qr.w);
       write_row<KC>(a, t, out_base(a, ci, qc.w, row, k));
     }

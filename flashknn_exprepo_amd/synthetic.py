@@ -11,7 +11,8 @@
   5 cm; density falls off as 1/r^2.
 * ``batch(seed)``: 16 clouds, sizes log-uniform in [2k, 120k], alternating
   room / LiDAR, per-cloud k in {8, 16, 32}.
-* ``hierarchy(seed)``: a room of ~100k points subsampled at 8/16/32 cm.
+* ``hierarchy(seed)``: a room of ~100k points subsampled at 12.5/25/50 cm
+  (~25k, ~6k, ~1.5k points).
 
 Voxel-first subsampling keeps, per voxel floor(xyz / size), the point with
 the smallest index (the deterministic form of the reference's
@@ -158,8 +159,9 @@ def pack(clouds):
     return xyz, off
 
 
-def hierarchy(seed=51, sizes=(0.08, 0.16, 0.32)):
-    """Room of ~100k points and its voxel-first subsampled stages."""
+def hierarchy(seed=51, sizes=(0.125, 0.25, 0.5)):
+    """Room of ~100k points and its voxel-first subsampled stages
+    (~100k -> ~25k -> ~6k -> ~1.5k)."""
     base = room(seed=seed)
     stages = [base]
     for s in sizes:
