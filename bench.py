@@ -87,7 +87,7 @@ def main():
     off = T(np.array([n], np.int32))
     K = 16
     eng = fk.ExactKNN()
-    cell = fk.cell_sizes(xyz, off, K)
+    cell = eng.build(xyz, off, K)          # density first guess + feedback
     inv = (1.0 / cell.double()).float()
     g = eng.grid
     log("build")
@@ -167,7 +167,7 @@ def main():
 
         # K = 32 on the same room
         e32 = fk.ExactKNN()
-        e32.grid.build_xyz(xyz, off, (1.0 / fk.cell_sizes(xyz, off, 32).double()).float())
+        e32.build(xyz, off, 32)
         v, ms = qps(e32.grid, 32, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
         extra["room_k32_Mqps"], extra["room_k32_ms"] = round(v, 2), round(ms, 4)
 
@@ -185,7 +185,7 @@ def main():
         lx = T(lid)
         lo = T(np.array([len(lid)], np.int32))
         el = fk.ExactKNN()
-        el.grid.build_xyz(lx, lo, (1.0 / fk.cell_sizes(lx, lo, K).double()).float())
+        el.build(lx, lo, K)
         v, ms = qps(el.grid, K, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
         extra["lidar_n"] = len(lid)
         extra["lidar_k16_Mqps"], extra["lidar_k16_ms"] = round(v, 2), round(ms, 4)
@@ -194,7 +194,7 @@ def main():
         uni = synthetic.uniform(8192, 47)
         ux, uo = T(uni), T(np.array([8192], np.int32))
         eu = fk.ExactKNN()
-        eu.grid.build_xyz(ux, uo, (1.0 / fk.cell_sizes(ux, uo, K).double()).float())
+        eu.build(ux, uo, K)
         v, ms = qps(eu.grid, K, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
         extra["uniform8192_k16_Mqps"], extra["uniform8192_k16_ms"] = round(v, 2), round(ms, 4)
 
@@ -213,7 +213,7 @@ def main():
             e = fk.ExactKNN()
             cx = T(cl)
             co = T(np.array([len(cl)], np.int32))
-            e.grid.build_xyz(cx, co, (1.0 / bk.cell[c:c + 1].double()).float())
+            e.build(cx, co, k, cell=bk.cell[c:c + 1])
             oi = torch.empty((len(cl), k), dtype=torch.int64, device=dev)
             od = torch.empty((len(cl), k), dtype=torch.float32, device=dev)
             engines.append((e.grid, k, oi, od, start))

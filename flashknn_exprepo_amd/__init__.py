@@ -18,12 +18,14 @@ from . import pointops
 from ._lib import FlashKNNError, load
 from .autograd import gathered_sqdist, idw_gather, idw_interpolate
 from .batched import BatchedKNN
-from .functions import FlashKNN, cal_recall, vanilla_Knn, xyz2key
+from .functions import (AdaptiveNeighborhoodFlashKNN, FlashKNN, build_adaptive_octree,
+                        cal_recall, select_adaptive_levels, vanilla_Knn, xyz2key)
 from .grid import Grid
 from .knn import ExactKNN, cell_sizes, knn
 
 __all__ = [
-    "FlashKNN", "xyz2key", "cal_recall", "vanilla_Knn", "pointops", "knn",
+    "FlashKNN", "AdaptiveNeighborhoodFlashKNN", "build_adaptive_octree",
+    "select_adaptive_levels", "xyz2key", "cal_recall", "vanilla_Knn", "pointops", "knn",
     "ExactKNN", "cell_sizes", "BatchedKNN", "Grid", "gathered_sqdist",
     "idw_gather", "idw_interpolate", "FlashKNNError", "load",
 ]

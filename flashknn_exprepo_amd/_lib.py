@@ -95,6 +95,10 @@ SIGNATURES = [
     ("fk_grid_build_cells", ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _vp]),
     ("fk_grid_build_xyz", ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _i32, _vp]),
     ("fk_grid_attach_queries", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _vp]),
+    ("fk_grid_auto_cell", ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _i32, ctypes.c_float,
+                                         _vp, _vp, _vp]),
+    ("fk_grid_adapt_cell", ctypes.c_int, [_vp, _vp, _i32, ctypes.c_float, ctypes.c_float,
+                                          _vp, _vp, _vp]),
     ("fk_grid_status", ctypes.c_int, [_vp, _vp]),
     ("fk_grid_info", ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32),
                                     ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
@@ -102,6 +106,7 @@ SIGNATURES = [
     ("fk_query", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp]),
     ("fk_query_stats", ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _vp]),
     ("fk_candidate_count", ctypes.c_int, [_vp, _i32, _i32, ctypes.POINTER(_i64), _vp]),
+    ("fk_point_candidates", ctypes.c_int, [_vp, _i32, _i32, _vp, _vp]),
     ("fk_batched_query", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp, _vp, _vp]),
     ("fk_batched_capture", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp, _vp, _vp]),
     ("fk_graph_replay", ctypes.c_int, [_vp, _vp]),
@@ -114,11 +119,16 @@ SIGNATURES = [
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Load and type libflashknn.so (raises if it was never built)."""
+def load(path: str = None):
+    """Load and type libflashknn.so (raises if it was never built).
+
+    FLASHKNN_LIB overrides the path (used to compare builds of the same
+    sources with different tuning knobs)."""
     global _lib
     if _lib is not None:
         return _lib
+    if path is None:
+        path = os.environ.get("FLASHKNN_LIB", LIB_PATH)
     if not os.path.exists(path):
         raise ImportError(
             f"{path} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "

@@ -2,11 +2,13 @@
 
 The kernel's cost per query is (candidates scanned) + (merges) + (the share
 of queries the first pass cannot certify, which pay a shell expansion).
-Larger cells certify more queries but scan more candidates; the best cell
-side depends on the cloud size, k and how the points fill space (surface
-scans vs volumes).  ``tune`` times the exact query for a few cell scales
-on the actual workload and records the fastest per (log2 N, k class) bucket
-in a JSON table; ``lookup_scale`` reads it (falling back to the default).
+Larger cells certify more queries but scan more candidates.  The cell side
+is set by density feedback to give ``target_ratio * k`` candidates per
+query in the 27-cell block; the best ratio depends on the cloud size, k and
+how the points fill space.  ``tune`` times the exact query for a few ratios
+on the actual workload and records the fastest per (log2 N, k class)
+bucket in a JSON table; ``lookup_target_ratio`` reads it (falling back to
+the default).
 """
 from __future__ import annotations
 
@@ -22,7 +24,7 @@ _TABLE = None
 TABLE_PATH = os.environ.get(
     "FLASHKNN_TUNE_TABLE",
     os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_table.json"))
-SCALES = (0.45, 0.6, 0.8, 1.05, 1.4)
+RATIOS = (3.0, 4.5, 6.0, 8.0, 11.0)
 
 
 def _kclass(k: int) -> int:
@@ -44,12 +46,12 @@ def _load():
     return _TABLE
 
 
-def lookup_scale(n: int, k: int) -> float:
-    from .knn import DEFAULT_CELL_SCALE
+def lookup_target_ratio(n: int, k: int) -> float:
+    from .knn import DEFAULT_TARGET_RATIO
     with _LOCK:
         t = _load()
     e = t.get(bucket(n, k))
-    return float(e["scale"]) if e else DEFAULT_CELL_SCALE
+    return float(e["target_ratio"]) if e else DEFAULT_TARGET_RATIO
 
 
 def _time_ms(fn, reps=5):
@@ -64,20 +66,19 @@ def _time_ms(fn, reps=5):
     return sorted(st[i].elapsed_time(st[i + 1]) for i in range(reps))[reps // 2]
 
 
-def tune(xyz, offsets, k, scales=SCALES, save=True):
-    """Time the exact query at each scale; record and return the best."""
-    from .knn import ExactKNN, cell_sizes
+def tune(xyz, offsets, k, ratios=RATIOS, save=True):
+    """Time the exact query at each target ratio; record and return the best."""
+    from .knn import ExactKNN
     eng = ExactKNN()
     results = {}
-    for s in scales:
-        cell = cell_sizes(xyz, offsets, k, s)
-        eng.grid.build_xyz(xyz, offsets, (1.0 / cell.double()).float())
-        results[s] = _time_ms(lambda: eng.grid.query(k, mode=1, pad_mode=1))
+    for r in ratios:
+        eng.build(xyz, offsets, k, target_ratio=r)
+        results[r] = _time_ms(lambda: eng.grid.query(k, mode=1, pad_mode=1))
     best = min(results, key=results.get)
     if save:
         with _LOCK:
             t = _load()
-            t[bucket(xyz.shape[0], k)] = {"scale": best, "ms": results}
+            t[bucket(xyz.shape[0], k)] = {"target_ratio": best, "ms": results}
             tmp = TABLE_PATH + ".tmp"
             with open(tmp, "w") as f:
                 json.dump(t, f, indent=1, sort_keys=True)

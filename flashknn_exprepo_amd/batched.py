@@ -16,14 +16,15 @@ import torch
 
 from . import _lib as F
 from .grid import Grid
-from .knn import cell_sizes
+from .autotune import lookup_target_ratio
+from .knn import DEFAULT_CELL_SCALE, adapt_rounds
 
 
 class BatchedKNN:
     """Exact (or 27-cell) kNN of every point of every cloud, per-cloud k."""
 
     def __init__(self, xyz, offsets, k_per_cloud, *, exact=True, cell=None,
-                 scale=None, idx_dtype=torch.int64):
+                 scale=None, target_ratio=None, idx_dtype=torch.int64):
         F.require_device(xyz, "xyz")
         self.xyz = xyz.float().contiguous()
         self.offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
@@ -45,11 +46,21 @@ class BatchedKNN:
         self.sizes = sizes
         self.out_idx = torch.empty(total, dtype=idx_dtype, device=xyz.device)
         self.out_dist = torch.empty(total, dtype=torch.float32, device=xyz.device)
-        if cell is None:
-            cell = cell_sizes(self.xyz, self.offsets, self.k, scale)
-        self.cell = cell
         self.grid = Grid()
-        self.grid.build_xyz(self.xyz, self.offsets, (1.0 / cell.double()).float())
+        if cell is None:
+            # first guess from each cloud's density, one feedback rebuild
+            side, inv = self.grid.auto_cell(self.xyz, self.offsets, self.k,
+                                            DEFAULT_CELL_SCALE if scale is None else scale)
+            self.grid.build_xyz(self.xyz, self.offsets, inv)
+            if target_ratio is None:
+                target_ratio = lookup_target_ratio(self.xyz.shape[0], self.max_k)
+            adapt_rounds(self.grid, self.xyz, self.offsets, self.k, side, inv,
+                         target_ratio)
+            cell = side
+        else:
+            cell = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
+            self.grid.build_xyz(self.xyz, self.offsets, (1.0 / cell.double()).float())
+        self.cell = cell
         self._captured = False
 
     def run(self):

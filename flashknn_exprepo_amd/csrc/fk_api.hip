@@ -81,6 +81,8 @@ struct fk_grid {
   CloudInfo *clouds = nullptr;
   int32_t *item_prefix = nullptr;
   int64_t cap_prefix = 0;
+  unsigned long long *cloud_acc = nullptr;
+  int64_t cap_acc = 0;
   // per point
   int64_t cap_points = 0, cap_p1 = 0, cap_p2 = 0, cap_p3 = 0, cap_p4 = 0;
   int4 *pt_cell = nullptr;
@@ -135,6 +137,7 @@ int ensure_clouds(fk_grid *g, int64_t B) {
   FK_CHECK(grow(g->bbox, g->cap_bbox, 6 * B, "bbox"));
   FK_CHECK(grow(g->clouds, g->cap_clouds, B, "clouds"));
   FK_CHECK(grow(g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
+  FK_CHECK(grow(g->cloud_acc, g->cap_acc, B, "per-cloud sums"));
   return FK_OK;
 }
 
@@ -310,7 +313,8 @@ int fk_grid_destroy(fk_grid *g) {
   };
   if (g->gexec) note(hipGraphExecDestroy(g->gexec));
   if (g->graph) note(hipGraphDestroy(g->graph));
-  void *bufs[] = {g->scalars, g->bbox, g->clouds, g->item_prefix, g->pt_cell,
+  void *bufs[] = {g->scalars, g->bbox, g->clouds, g->item_prefix, g->cloud_acc,
+                  g->pt_cell,
                   g->pt_lin, g->pt_rank, g->srec, g->scell, g->counts,
                   g->cell_start, g->qcounts, g->qstart, g->scan_part, g->qrec,
                   g->qcell, g->q_cell_abs, g->qpad, g->q_lin, g->q_rank,
@@ -580,6 +584,68 @@ int fk_candidate_count(fk_grid *g, int32_t radius_cells, int32_t query_set,
   FK_HIP(hipMemcpyAsync(&h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
   FK_HIP(hipStreamSynchronize(s));
   *total = (int64_t)h;
+  return ok();
+}
+
+int fk_point_candidates(fk_grid *g, int32_t radius_cells, int32_t query_set,
+                        int32_t *out, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  fk_query_params p;
+  memset(&p, 0, sizeof(p));
+  p.k = 1;
+  p.mode = FK_MODE_CELLS;
+  p.radius_cells = radius_cells;
+  p.cut_r2 = INFINITY;
+  p.pad_mode = query_set == FK_QUERIES_SELF ? FK_PAD_SELF : FK_PAD_MINUS1;
+  p.query_set = query_set;
+  QueryArgs a;
+  int dummy = 0;
+  FK_CHECK(make_args(g, &p, &dummy, nullptr, a));
+  if (!out && a.nq > 0) return fail(FK_ERR_INVALID_ARG, "fk_point_candidates: out is NULL");
+  FK_HIP(launch_point_candidates(a, out, S(stream)));
+  return ok();
+}
+
+int fk_grid_auto_cell(fk_grid *g, const float *xyz, const int32_t *offsets,
+                      int32_t num_clouds, int64_t n, const int32_t *k_per_cloud,
+                      int32_t k, float scale, float *side_out, float *inv_out,
+                      void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (n < 0 || num_clouds < 1 || !offsets || !inv_out || (n > 0 && !xyz) ||
+      (!k_per_cloud && (k < 1 || k > FK_MAX_K)) || !(scale > 0.f))
+    return fail(FK_ERR_INVALID_ARG, "fk_grid_auto_cell: bad arguments");
+  if (num_clouds > g->cap_clouds) {
+    if (g->reserved)
+      return fail(FK_ERR_CAPACITY, "fk_grid_auto_cell: more clouds than reserved");
+    FK_CHECK(ensure_clouds(g, num_clouds));
+  }
+  FK_HIP(launch_auto_cell(g->bbox, xyz, offsets, num_clouds, n, k_per_cloud, k,
+                          scale, side_out, inv_out, S(stream)));
+  return ok();
+}
+
+int fk_grid_adapt_cell(fk_grid *g, const int32_t *k_per_cloud, int32_t k,
+                       float target_ratio, float exponent, float *side_io,
+                       float *inv_io, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (!g->built || !g->from_xyz)
+    return fail(FK_ERR_STATE, "fk_grid_adapt_cell needs a grid built from xyz");
+  if (!side_io || !inv_io || !(target_ratio > 0.f) || !(exponent > 0.f) ||
+      (!k_per_cloud && (k < 1 || k > FK_MAX_K)))
+    return fail(FK_ERR_INVALID_ARG, "fk_grid_adapt_cell: bad arguments");
+  fk_query_params p;
+  memset(&p, 0, sizeof(p));
+  p.k = 1;
+  p.mode = FK_MODE_CELLS;
+  p.radius_cells = 1;
+  p.cut_r2 = INFINITY;
+  p.query_set = FK_QUERIES_SELF;
+  QueryArgs a;
+  int dummy = 0;
+  FK_CHECK(make_args(g, &p, &dummy, nullptr, a));
+  a.num_clouds = g->n_clouds;
+  FK_HIP(launch_adapt_cell(a, g->cloud_acc, k_per_cloud, k, target_ratio,
+                           exponent, side_io, inv_io, S(stream)));
   return ok();
 }
 

@@ -93,29 +93,105 @@ __device__ __forceinline__ int wave_max(int v) {
   return v;
 }
 
-// Per-cloud bounding box of cell coordinates.  When all valid lanes of the
-// wave belong to one cloud (the usual case) the wave reduces first and one
-// lane issues the six atomics.
-__device__ void bbox_update(int32_t *bbox, int cloud, int cx, int cy, int cz,
-                            bool valid) {
-  const uint64_t vm = __ballot(valid);
-  if (vm == 0ull) return;
-  const int leader = __builtin_ctzll(vm);
-  const int c0 = __shfl(cloud, leader, kWave);
-  if (wave_all(!valid || cloud == c0)) {
-    const int lx = wave_min(valid ? cx : INT_MAX), hx = wave_max(valid ? cx : INT_MIN);
-    const int ly = wave_min(valid ? cy : INT_MAX), hy = wave_max(valid ? cy : INT_MIN);
-    const int lz = wave_min(valid ? cz : INT_MAX), hz = wave_max(valid ? cz : INT_MIN);
-    if ((int)(threadIdx.x & (kWave - 1)) == leader) {
-      int32_t *b = bbox + 6 * c0;
-      atomicMin(b + 0, lx); atomicMin(b + 1, ly); atomicMin(b + 2, lz);
-      atomicMax(b + 3, hx); atomicMax(b + 4, hy); atomicMax(b + 5, hz);
-    }
-  } else if (valid) {
-    int32_t *b = bbox + 6 * cloud;
-    atomicMin(b + 0, cx); atomicMin(b + 1, cy); atomicMin(b + 2, cz);
-    atomicMax(b + 3, cx); atomicMax(b + 4, cy); atomicMax(b + 5, cz);
+// Per-cloud bounding boxes.  Each thread accumulates kItems points, the wave
+// and then the block reduce, so a block whose points all belong to one cloud
+// issues six atomics in total (same-address atomics serialise in L2: one
+// set per wave cost ~0.1 ms per 100k points).  Values are ints; float
+// coordinates go through the monotone mapping ord_of.
+constexpr int kItems = 4;
+constexpr int kMaxWaves = 16;
+
+struct Box {
+  int cloud;  // -1: nothing accumulated yet
+  int lo[3], hi[3];
+};
+
+__device__ __forceinline__ void box_init(Box &b) {
+  b.cloud = -1;
+  b.lo[0] = b.lo[1] = b.lo[2] = INT_MAX;
+  b.hi[0] = b.hi[1] = b.hi[2] = INT_MIN;
+}
+
+__device__ __forceinline__ void box_atomic(int32_t *bbox, int c, const int *lo,
+                                           const int *hi) {
+  int32_t *p = bbox + 6 * c;
+  atomicMin(p + 0, lo[0]); atomicMin(p + 1, lo[1]); atomicMin(p + 2, lo[2]);
+  atomicMax(p + 3, hi[0]); atomicMax(p + 4, hi[1]); atomicMax(p + 5, hi[2]);
+}
+
+// A point of a cloud other than the one being accumulated goes straight to
+// memory (happens only at cloud boundaries).
+__device__ __forceinline__ void box_add(Box &b, int32_t *bbox, int c, int x,
+                                        int y, int z) {
+  if (b.cloud < 0) b.cloud = c;
+  if (c == b.cloud) {
+    b.lo[0] = min(b.lo[0], x); b.lo[1] = min(b.lo[1], y); b.lo[2] = min(b.lo[2], z);
+    b.hi[0] = max(b.hi[0], x); b.hi[1] = max(b.hi[1], y); b.hi[2] = max(b.hi[2], z);
+  } else {
+    const int v[3] = {x, y, z};
+    box_atomic(bbox, c, v, v);
   }
+}
+
+// Called by every thread of the block.
+__device__ void box_flush_block(Box &b, int32_t *bbox) {
+  __shared__ int s_cloud[kMaxWaves];
+  __shared__ int s_lo[kMaxWaves][3], s_hi[kMaxWaves][3];
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const bool has = b.cloud >= 0;
+  const uint64_t m = __ballot(has);
+  int wc = -1;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (m != 0ull) {
+    const int c0 = __shfl(b.cloud, (int)__builtin_ctzll(m), kWave);
+    if (wave_all(!has || b.cloud == c0)) {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = wave_min(has ? b.lo[a] : INT_MAX);
+        hi[a] = wave_max(has ? b.hi[a] : INT_MIN);
+      }
+      wc = c0;
+    } else if (has) {
+      box_atomic(bbox, b.cloud, b.lo, b.hi);
+    }
+  }
+  if (lane == 0) {
+    s_cloud[w] = wc;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { s_lo[w][a] = lo[a]; s_hi[w][a] = hi[a]; }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int cur = -1;
+    int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+    for (int j = 0; j < nw; ++j) {
+      const int c = s_cloud[j];
+      if (c < 0) continue;
+      if (c != cur) {
+        if (cur >= 0) box_atomic(bbox, cur, clo, chi);
+        cur = c;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { clo[a] = s_lo[j][a]; chi[a] = s_hi[j][a]; }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          clo[a] = min(clo[a], s_lo[j][a]);
+          chi[a] = max(chi[a], s_hi[j][a]);
+        }
+      }
+    }
+    if (cur >= 0) box_atomic(bbox, cur, clo, chi);
+  }
+}
+
+// Monotone float -> int map (for non-NaN values) and its inverse.
+__device__ __forceinline__ int ord_of(float f) {
+  const int i = __float_as_int(f);
+  return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+__device__ __forceinline__ float float_of_ord(int i) {
+  return __int_as_float(i >= 0 ? i : (i ^ 0x7fffffff));
 }
 
 __global__ void k_init_bbox(int32_t *bbox, int32_t B) {
@@ -132,27 +208,30 @@ __device__ __forceinline__ bool cell_ok(int64_t v) {
 }
 
 // cell = grid >> num_down (arithmetic shift = floor division by 2^num_down)
-__global__ void k_cells_from_grid(const int64_t *grid, const int64_t *batch,
-                                  int64_t n, int32_t num_down, int32_t B,
-                                  int4 *pt_cell, int32_t *bbox,
-                                  int32_t *scalars) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  bool valid = i < n;
-  int cx = 0, cy = 0, cz = 0, cl = 0;
-  if (valid) {
-    const int64_t gx = grid[3 * i] >> num_down, gy = grid[3 * i + 1] >> num_down,
-                  gz = grid[3 * i + 2] >> num_down;
-    const int64_t b = batch ? batch[i] : 0;
-    if (b < 0 || b >= B || !cell_ok(gx) || !cell_ok(gy) || !cell_ok(gz)) {
-      raise_status(scalars, FK_ERR_DOMAIN);
-      valid = false;
-      pt_cell[i] = make_int4(0, 0, 0, -1);
-    } else {
-      cx = (int)gx; cy = (int)gy; cz = (int)gz; cl = (int)b;
-      pt_cell[i] = make_int4(cx, cy, cz, cl);
+__global__ void __launch_bounds__(256)
+k_cells_from_grid(const int64_t *grid, const int64_t *batch, int64_t n,
+                  int32_t num_down, int32_t B, int4 *pt_cell, int32_t *bbox,
+                  int32_t *scalars) {
+  Box box;
+  box_init(box);
+  const int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
+    if (i < n) {
+      const int64_t gx = grid[3 * i] >> num_down, gy = grid[3 * i + 1] >> num_down,
+                    gz = grid[3 * i + 2] >> num_down;
+      const int64_t b = batch ? batch[i] : 0;
+      if (b < 0 || b >= B || !cell_ok(gx) || !cell_ok(gy) || !cell_ok(gz)) {
+        raise_status(scalars, FK_ERR_DOMAIN);
+        pt_cell[i] = make_int4(0, 0, 0, -1);
+      } else {
+        pt_cell[i] = make_int4((int)gx, (int)gy, (int)gz, (int)b);
+        box_add(box, bbox, (int)b, (int)gx, (int)gy, (int)gz);
+      }
     }
   }
-  bbox_update(bbox, cl, cx, cy, cz, valid);
+  box_flush_block(box, bbox);
 }
 
 // first c with offsets[c] > i  (offsets = cumulative segment ends)
@@ -172,37 +251,92 @@ __device__ __forceinline__ float scaled(float x, float inv) {
   return __fmul_rn(x, inv);
 }
 
-__global__ void k_cells_from_xyz(const float *xyz, const int32_t *offsets,
-                                 int32_t B, int64_t n, const float *inv_cell,
-                                 int4 *pt_cell, int32_t *bbox,
-                                 int32_t *scalars) {
-  const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  bool valid = i < n;
-  int cx = 0, cy = 0, cz = 0, cl = 0;
-  if (valid) {
-    cl = cloud_of(offsets, B, i);
-    bool ok = cl < B;
-    float fx = 0.f, fy = 0.f, fz = 0.f;
-    if (ok) {
-      const float inv = inv_cell[cl];
-      fx = floorf(scaled(xyz[3 * i], inv));
-      fy = floorf(scaled(xyz[3 * i + 1], inv));
-      fz = floorf(scaled(xyz[3 * i + 2], inv));
-      // rejects NaN as well
-      ok = fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
-           fabsf(fz) < (float)kCellLimit;
-    }
-    if (!ok) {
-      raise_status(scalars, FK_ERR_DOMAIN);
-      valid = false;
-      cl = 0;
-      pt_cell[i] = make_int4(0, 0, 0, -1);
-    } else {
-      cx = (int)fx; cy = (int)fy; cz = (int)fz;
-      pt_cell[i] = make_int4(cx, cy, cz, cl);
+__global__ void __launch_bounds__(256)
+k_cells_from_xyz(const float *xyz, const int32_t *offsets, int32_t B, int64_t n,
+                 const float *inv_cell, int4 *pt_cell, int32_t *bbox,
+                 int32_t *scalars) {
+  Box box;
+  box_init(box);
+  const int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
+    if (i < n) {
+      const int cl = cloud_of(offsets, B, i);
+      bool ok = cl < B;
+      float fx = 0.f, fy = 0.f, fz = 0.f;
+      if (ok) {
+        const float inv = inv_cell[cl];
+        fx = floorf(scaled(xyz[3 * i], inv));
+        fy = floorf(scaled(xyz[3 * i + 1], inv));
+        fz = floorf(scaled(xyz[3 * i + 2], inv));
+        // rejects NaN as well
+        ok = fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
+             fabsf(fz) < (float)kCellLimit;
+      }
+      if (!ok) {
+        raise_status(scalars, FK_ERR_DOMAIN);
+        pt_cell[i] = make_int4(0, 0, 0, -1);
+      } else {
+        pt_cell[i] = make_int4((int)fx, (int)fy, (int)fz, cl);
+        box_add(box, bbox, cl, (int)fx, (int)fy, (int)fz);
+      }
     }
   }
-  bbox_update(bbox, cl, cx, cy, cz, valid);
+  box_flush_block(box, bbox);
+}
+
+// Float bounding box of every cloud (ord_of-mapped ints in bbox[6c..6c+5]).
+__global__ void __launch_bounds__(256)
+k_cloud_bounds(const float *xyz, const int32_t *offsets, int32_t B, int64_t n,
+               int32_t *bbox) {
+  Box box;
+  box_init(box);
+  const int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems;
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
+    if (i < n) {
+      const int cl = cloud_of(offsets, B, i);
+      const float x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+      if (cl < B && x == x && y == y && z == z)
+        box_add(box, bbox, cl, ord_of(x), ord_of(y), ord_of(z));
+    }
+  }
+  box_flush_block(box, bbox);
+}
+
+// Cell side per cloud from its bounding-box density:
+// side = scale * (k * volume / n)^(1/3), degenerate axes counted with 1% of
+// the largest extent, and at least (volume / (64 n))^(1/3) so that the dense
+// cell box holds at most ~64 cells per point.
+__global__ void k_auto_cell(const int32_t *bbox, const int32_t *offsets,
+                            int32_t B, const int32_t *k_per_cloud, int32_t k,
+                            float scale, float *side_out, float *inv_out) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B) return;
+  const int32_t *b = bbox + 6 * c;
+  const int64_t n = offsets[c] - (c ? offsets[c - 1] : 0);
+  double side = 1.0;
+  if (n > 0 && b[0] <= b[3]) {
+    double ext[3], span = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      ext[a] = (double)float_of_ord(b[3 + a]) - (double)float_of_ord(b[a]);
+      span = fmax(span, ext[a]);
+    }
+    span = fmax(span, 1e-6);
+    double vol = 1.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) vol *= fmax(ext[a], 1e-2 * span);
+    const double kk = (double)(k_per_cloud ? k_per_cloud[c] : k);
+    side = (double)scale * cbrt(kk * vol / (double)n);
+    side = fmax(side, cbrt(vol / (64.0 * (double)n)));
+    side = fmax(side, 1e-6);
+  }
+  const float s = (float)side;
+  if (side_out) side_out[c] = s;
+  inv_out[c] = (float)(1.0 / (double)s);
 }
 
 // One block: per-cloud box sizes, exclusive scan of cell counts, CloudInfo.
@@ -534,9 +668,9 @@ hipError_t launch_build_cells(const BuildBuffers &b, const float *xyz,
   hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s,
                      b.bbox, B);
   if (n > 0)
-    hipLaunchKernelGGL(k_cells_from_grid, dim3(blocks_for(n)), dim3(kBlock), 0,
-                       s, grid, batch, n, num_down, B, b.pt_cell, b.bbox,
-                       b.scalars);
+    hipLaunchKernelGGL(k_cells_from_grid, dim3(blocks_for(n, kBlock * kItems)),
+                       dim3(kBlock), 0, s, grid, batch, n, num_down, B,
+                       b.pt_cell, b.bbox, b.scalars);
   return hipGetLastError();
 }
 
@@ -546,9 +680,22 @@ hipError_t launch_build_xyz_cells(const BuildBuffers &b, const float *xyz,
   hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s,
                      b.bbox, B);
   if (n > 0)
-    hipLaunchKernelGGL(k_cells_from_xyz, dim3(blocks_for(n)), dim3(kBlock), 0,
-                       s, xyz, offsets, B, n, inv_cell, b.pt_cell, b.bbox,
-                       b.scalars);
+    hipLaunchKernelGGL(k_cells_from_xyz, dim3(blocks_for(n, kBlock * kItems)),
+                       dim3(kBlock), 0, s, xyz, offsets, B, n, inv_cell,
+                       b.pt_cell, b.bbox, b.scalars);
+  return hipGetLastError();
+}
+
+hipError_t launch_auto_cell(int32_t *bbox, const float *xyz,
+                            const int32_t *offsets, int32_t B, int64_t n,
+                            const int32_t *k_per_cloud, int32_t k, float scale,
+                            float *side_out, float *inv_out, hipStream_t s) {
+  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s, bbox, B);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cloud_bounds, dim3(blocks_for(n, kBlock * kItems)),
+                       dim3(kBlock), 0, s, xyz, offsets, B, n, bbox);
+  hipLaunchKernelGGL(k_auto_cell, dim3(blocks_for(B)), dim3(kBlock), 0, s, bbox,
+                     offsets, B, k_per_cloud, k, scale, side_out, inv_out);
   return hipGetLastError();
 }
 

@@ -82,6 +82,10 @@ hipError_t launch_build_xyz_cells(const BuildBuffers &b, const float *xyz,
                                   const int32_t *offsets, int32_t num_clouds,
                                   int64_t n, const float *inv_cell,
                                   hipStream_t s);
+hipError_t launch_auto_cell(int32_t *bbox, const float *xyz,
+                            const int32_t *offsets, int32_t B, int64_t n,
+                            const int32_t *k_per_cloud, int32_t k, float scale,
+                            float *side_out, float *inv_out, hipStream_t s);
 hipError_t launch_layout(const BuildBuffers &b, int32_t num_clouds,
                          const int32_t *offsets, const float *inv_cell,
                          hipStream_t s);
@@ -144,6 +148,15 @@ hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s);
 int kclass_of(int k);  // 8, 16, 32 or 64
 hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
                                   hipStream_t s);
+hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,
+                                   hipStream_t s);
+// Per-cloud candidate sums of the self queries (per_cloud: u64[num_clouds],
+// zeroed here), then side *= clamp((target_ratio*k / mean)^(1/exponent))
+// and inv = 1/side, in place on the device.
+hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *per_cloud,
+                             const int32_t *k_per_cloud, int32_t k,
+                             float target_ratio, float exponent, float *side_io,
+                             float *inv_io, hipStream_t s);
 hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
                         int32_t B, int32_t *prefix, int32_t *scalars,
                         hipStream_t s);

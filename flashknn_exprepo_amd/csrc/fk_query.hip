@@ -36,8 +36,45 @@ namespace fk {
 
 namespace {
 
-constexpr int kUnroll = 4;
+// Tuning knobs (compile time; the defaults are the measured best):
+//   FK_UNROLL        candidates loaded per lane per step
+//   FK_HALF_MASK     bit 0/1/2: K class 16/32/64 buffers KC/2 keys (merged
+//                    with a half-cleaner) instead of KC
+//   FK_WPE_<KC>      minimum waves per SIMD requested for the first pass
+#ifndef FK_UNROLL
+#define FK_UNROLL 4
+#endif
+#ifndef FK_HALF_MASK
+#define FK_HALF_MASK 0
+#endif
+#ifndef FK_WPE_8
+#define FK_WPE_8 1
+#endif
+#ifndef FK_WPE_16
+#define FK_WPE_16 1
+#endif
+#ifndef FK_WPE_32
+#define FK_WPE_32 2
+#endif
+#ifndef FK_WPE_64
+#define FK_WPE_64 1
+#endif
+
+constexpr int kUnroll = FK_UNROLL;
 constexpr uint64_t kMaxKey = ~0ull;
+
+template <int KC>
+constexpr int buffer_slots() {
+  return ((KC == 16 && (FK_HALF_MASK & 1)) || (KC == 32 && (FK_HALF_MASK & 2)) ||
+          (KC == 64 && (FK_HALF_MASK & 4)))
+             ? KC / 2
+             : KC;
+}
+
+template <int KC>
+constexpr int min_waves_per_eu() {
+  return KC == 8 ? FK_WPE_8 : KC == 16 ? FK_WPE_16 : KC == 32 ? FK_WPE_32 : FK_WPE_64;
+}
 constexpr uint32_t kInfBits = 0x7f800000u;
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
@@ -127,19 +164,37 @@ struct TopK {
 
   // Merge the buffered keys into the sorted list (uniform work per lane).
   __device__ __forceinline__ void flush() {
-    uint64_t b[KC];
+    constexpr int B = buffer_slots<KC>();
+    uint64_t b[B];
 #pragma unroll
-    for (int i = 0; i < KC; ++i) {
+    for (int i = 0; i < B; ++i) {
       const uint64_t v = lb[i * kWave];
       b[i] = i < cnt ? v : kMaxKey;
     }
-    bitonic_sort<KC>(b);
+    bitonic_sort<B>(b);
+    if constexpr (B == KC) {
+      // min(top[i], b[KC-1-i]) is bitonic and holds the KC smallest
 #pragma unroll
-    for (int i = 0; i < KC; ++i) {
-      const uint64_t o = b[KC - 1 - i];
-      top[i] = o < top[i] ? o : top[i];
+      for (int i = 0; i < KC; ++i) {
+        const uint64_t o = b[KC - 1 - i];
+        top[i] = o < top[i] ? o : top[i];
+      }
+      bitonic_merge<KC>(top);
+    } else {
+      // B = KC/2: the half-cleaner min(top[B+i], b[B-1-i]) gives the B
+      // smallest of the upper half and the buffer (bitonic); sort it, then
+      // merge the two sorted halves (second half read reversed = bitonic).
+      uint64_t h[B];
+#pragma unroll
+      for (int i = 0; i < B; ++i) {
+        const uint64_t o = b[B - 1 - i];
+        h[i] = o < top[B + i] ? o : top[B + i];
+      }
+      bitonic_merge<B>(h);
+#pragma unroll
+      for (int i = 0; i < B; ++i) top[KC - 1 - i] = h[i];
+      bitonic_merge<KC>(top);
     }
-    bitonic_merge<KC>(top);
     cnt = 0;
     refresh_thr();
   }
@@ -162,7 +217,7 @@ __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, float qx,
                                            const float4 *__restrict__ srec) {
   const int len = e - b;
   for (int s = 0; wave_any(s < len); s += kUnroll) {
-    if (wave_any(t.cnt > KC - kUnroll)) t.flush();
+    if (wave_any(t.cnt > buffer_slots<KC>() - kUnroll)) t.flush();
     float4 r[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -192,48 +247,28 @@ __device__ __forceinline__ void column_range(const CloudInfo &ci,
   e = cs[col + z1 + 1];
 }
 
-// Scan the full block of half-width R around the query's cell.
+// Scan the full block of half-width R around the query's cell.  The
+// query's own column goes first: its points are the nearest on average, so
+// the k-th distance (the acceptance threshold) is tight early and fewer
+// later candidates get buffered and merged.  Then the others in raster
+// order (dx, dy) = (0, 0) is skipped.
 template <int KC>
 __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
                                            float4 qr, int R, bool active) {
+  {
+    int b = 0, e = 0;
+    if (active)
+      column_range(ci, a.cell_start, qc.x, qc.y, qc.z - R, qc.z + R, b, e);
+    scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
+  }
   for (int dx = -R; dx <= R; ++dx) {
     for (int dy = -R; dy <= R; ++dy) {
+      if (dx == 0 && dy == 0) continue;
       int b = 0, e = 0;
       if (active)
         column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - R,
                      qc.z + R, b, e);
-      scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
-    }
-  }
-}
-
-// Scan the shell Ro < max(|dx|,|dy|,|dz|) <= Rn (divergent; pending queries).
-template <int KC>
-__device__ void scan_shell(TopK<KC> &t, const QueryArgs &a, const CloudInfo &ci,
-                           int4 qc, float4 qr, int Ro, int Rn) {
-  // columns clamped to the box; z bounds in 64 bits, clamped to [-1, nz]
-  // (column_range clips them to the box), so no int overflow for queries
-  // far outside their cloud's box.
-  const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
-  const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
-  auto zc = [&](long long v) {
-    return (int)(v < -1 ? -1 : (v > ci.nz ? ci.nz : v));
-  };
-  const long long zq = qc.z;
-  const int zlo = zc(zq - Rn), zhi = zc(zq + Rn);
-  const int zin_lo = zc(zq - Ro - 1), zin_hi = zc(zq + Ro + 1);
-  for (int dx = x0; dx <= x1; ++dx) {
-    for (int dy = y0; dy <= y1; ++dy) {
-      const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
-      int b, e;
-      if (inner) {
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zin_lo, b, e);
-        scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zin_hi, zhi, b, e);
-      } else {
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zhi, b, e);
-      }
       scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
     }
   }
@@ -417,73 +452,204 @@ constexpr int waves_per_block() {
 }
 
 template <int KC>
-__global__ void __launch_bounds__(64 * waves_per_block<KC>())
+__global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
 k_query(QueryArgs a) {
   constexpr int WPB = waves_per_block<KC>();
-  __shared__ uint64_t sbuf[WPB * KC * kWave];
+  constexpr int BS = buffer_slots<KC>();
+  __shared__ uint64_t sbuf[WPB * BS * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   const int q0 = (blk * WPB + w) * kWave;
   // wave-uniform exits: past the end, or the build/attach raised an error
   // (its tables may then be incomplete)
   if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
-  run_tile<KC>(a, q0, a.nq, sbuf + w * KC * kWave + lane);
+  run_tile<KC>(a, q0, a.nq, sbuf + w * BS * kWave + lane);
 }
 
-// Pending queries of FK_MODE_EXACT: grow the block until certified.
-template <int KC>
-__global__ void __launch_bounds__(64 * waves_per_block<KC>())
-k_query_expand(QueryArgs a) {
-  constexpr int WPB = waves_per_block<KC>();
-  __shared__ uint64_t sbuf[WPB * KC * kWave];
+// ------------------------------------------------- cooperative expansion
+//
+// Pending queries of FK_MODE_EXACT (not certified by the first pass): one
+// wave per query.  The query's best-k list is spread over the wave, lane i
+// holding the i-th best key; the 64 lanes split the shell's columns between
+// them, append the candidates below the current k-th key to a per-wave LDS
+// buffer, and every 64 buffered keys are merged by a wave-wide bitonic sort
+// and merge through lane shuffles.  The block then grows shell by shell
+// (radius from the current k-th distance, doubled while it is infinite)
+// until the query certifies or the block covers its cloud's box.
+
+constexpr int kCoopWaves = 4;          // waves per block
+constexpr int kCoopCap = 256;          // buffered keys per wave
+
+__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
+__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
+
+// Ascending bitonic sort of one key per lane across the wave.
+__device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
+#pragma unroll
+  for (int k2 = 2; k2 <= kWave; k2 <<= 1) {
+#pragma unroll
+    for (int j = k2 >> 1; j > 0; j >>= 1) {
+      const uint64_t o = __shfl_xor(v, j, kWave);
+      const bool up = (lane & k2) == 0;
+      const bool lower = (lane & j) == 0;
+      v = (lower == up) ? umin64(v, o) : umax64(v, o);
+    }
+  }
+  return v;
+}
+
+// L and S ascending across lanes -> the 64 smallest of both, ascending.
+__device__ __forceinline__ uint64_t wave_merge(uint64_t L, uint64_t S, int lane) {
+  uint64_t v = umin64(L, __shfl(S, kWave - 1 - lane, kWave));
+#pragma unroll
+  for (int j = kWave >> 1; j > 0; j >>= 1) {
+    const uint64_t o = __shfl_xor(v, j, kWave);
+    v = (lane & j) == 0 ? umin64(v, o) : umax64(v, o);
+  }
+  return v;
+}
+
+struct CoopList {
+  uint64_t L;       // this lane's rank in the sorted best list
+  uint64_t *buf;    // the wave's LDS buffer
+  int cnt;          // buffered keys (wave-uniform)
+  int k;
+  uint64_t cutkey, thr;
+
+  __device__ __forceinline__ void refresh(int lane) {
+    (void)lane;
+    const uint64_t kth = __shfl(L, k - 1, kWave);
+    thr = kth < cutkey ? kth : cutkey;
+  }
+  __device__ __forceinline__ void flush(int lane) {
+    for (int c0 = 0; c0 < cnt; c0 += kWave) {
+      const uint64_t v = c0 + lane < cnt ? buf[c0 + lane] : kMaxKey;
+      L = wave_merge(L, wave_sort(v, lane), lane);
+    }
+    cnt = 0;
+    refresh(lane);
+  }
+  __device__ __forceinline__ void offer(uint64_t key, bool has, int lane) {
+    const bool acc = has && key < thr;
+    const uint64_t m = __ballot(acc);
+    if (m != 0ull) {
+      const uint64_t below = lane ? (m & (~0ull >> (64 - lane))) : 0ull;
+      if (acc) buf[cnt + __popcll(below)] = key;
+      cnt += __popcll(m);
+      if (cnt > kCoopCap - kWave) flush(lane);
+    }
+  }
+};
+
+// Columns of the shell Ro < Chebyshev distance <= Rn, split over the lanes.
+__device__ void coop_shell(CoopList &t, const QueryArgs &a, const CloudInfo &ci,
+                           int4 qc, float4 qr, int Ro, int Rn, int lane) {
+  const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
+  const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
+  const long long wy = y1 >= y0 ? (long long)(y1 - y0 + 1) : 0;
+  const long long ncol = x1 >= x0 ? (long long)(x1 - x0 + 1) * wy : 0;
+  auto zc = [&](long long v) {
+    return (int)(v < -1 ? -1 : (v > ci.nz ? ci.nz : v));
+  };
+  const long long zq = qc.z;
+  const int zlo = zc(zq - Rn), zhi = zc(zq + Rn);
+  const int zin_lo = zc(zq - Ro - 1), zin_hi = zc(zq + Ro + 1);
+  long long s = lane;  // this lane's next column
+  int seg = 0;         // inner columns have two z segments
+  int p = 0, e = 0;
+  auto advance = [&]() {
+    while (p >= e && s < ncol) {
+      const int dx = x0 + (int)(s / wy), dy = y0 + (int)(s % wy);
+      const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
+      if (inner) {
+        if (seg == 0)
+          column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zin_lo, p, e);
+        else
+          column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zin_hi, zhi, p, e);
+        if (seg == 0) {
+          seg = 1;
+        } else {
+          seg = 0;
+          s += kWave;
+        }
+      } else {
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zhi, p, e);
+        s += kWave;
+      }
+    }
+  };
+  advance();
+  while (wave_any(p < e)) {
+    const bool has = p < e;
+    uint64_t key = kMaxKey;
+    if (has) {
+      const float4 r = a.srec[p];
+      key = make_key(sqdist(r.x, r.y, r.z, qr.x, qr.y, qr.z), __float_as_uint(r.w));
+      ++p;
+    }
+    t.offer(key, has, lane);
+    if (p >= e) advance();
+  }
+  if (t.cnt > 0) t.flush(lane);
+}
+
+__global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
+  __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.scalars[kPending], a.nq);
-  const int nwaves = gridDim.x * WPB;
-  for (int base = (blockIdx.x * WPB + w) * kWave; base < npend;
-       base += nwaves * kWave) {
-    const int slot = base + lane;
-    const bool active = slot < npend;
-    const int qs = a.pend_q[active ? slot : base];
+  const int nwaves = gridDim.x * kCoopWaves;
+  int nexp = 0;
+  for (int slot = blockIdx.x * kCoopWaves + w; slot < npend; slot += nwaves) {
+    const int qs = a.pend_q[slot];
     const float4 qr = a.qrec[qs];
     const int4 qc = a.qcell[qs];
     const CloudInfo ci = a.clouds[qc.w];
     const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
-    TopK<KC> t;
-    t.init(sbuf + w * KC * kWave + lane, k, a.cut_r2, pad_value(a, qr, qs));
-    const int64_t pb = (int64_t)(active ? slot : base) * a.k - t.base;
-#pragma unroll
-    for (int j = 0; j < KC; ++j)
-      if (j >= t.base) t.top[j] = a.pend_keys[pb + j];
-    t.refresh_thr();
+    CoopList t;
+    t.buf = sbuf + w * kCoopCap;
+    t.cnt = 0;
+    t.k = k;
+    t.cutkey = make_key(a.cut_r2, 0u);
+    t.L = lane < k ? a.pend_keys[(int64_t)slot * a.k + lane] : kMaxKey;
+    t.refresh(lane);
     const int Rcover = cover_radius(ci, qc);
     int Ro = a.R;
-    bool done = !active || Ro >= Rcover;
-    while (wave_any(!done)) {
-      if (!done) {
-        const uint64_t kk = t.kth();
-        float eff = __uint_as_float((uint32_t)(kk >> 32));
-        eff = eff < a.cut_r2 ? eff : a.cut_r2;
-        const int Rn = next_radius(ci, qc, qr, Ro, eff, Rcover);
-        scan_shell<KC>(t, a, ci, qc, qr, Ro, Rn);
-        if (t.cnt > 0) t.flush();
-        Ro = Rn;
-        double umax;
-        const double gap = cert_gap(ci, qc, qr, Ro, umax);
-        const uint64_t k2 = t.kth();
-        float eff2 = __uint_as_float((uint32_t)(k2 >> 32));
-        eff2 = eff2 < a.cut_r2 ? eff2 : a.cut_r2;
-        done = Ro >= Rcover || certified(gap, eff2, ci.inv_cell, umax, Ro);
+    while (Ro < Rcover) {  // wave-uniform
+      const uint64_t kk = __shfl(t.L, k - 1, kWave);
+      float eff = __uint_as_float((uint32_t)(kk >> 32));
+      eff = eff < a.cut_r2 ? eff : a.cut_r2;
+      const int Rn = next_radius(ci, qc, qr, Ro, eff, Rcover);
+      coop_shell(t, a, ci, qc, qr, Ro, Rn, lane);
+      Ro = Rn;
+      double umax;
+      const double gap = cert_gap(ci, qc, qr, Ro, umax);
+      const uint64_t k2 = __shfl(t.L, k - 1, kWave);
+      float eff2 = __uint_as_float((uint32_t)(k2 >> 32));
+      eff2 = eff2 < a.cut_r2 ? eff2 : a.cut_r2;
+      if (certified(gap, eff2, ci.inv_cell, umax, Ro)) break;
+    }
+    ++nexp;
+    if (lane < k) {
+      const int row = __float_as_int(qr.w);
+      const int64_t o = out_base(a, ci, qc.w, row, k) + lane;
+      const uint64_t key = t.L;
+      const int id = (int)(uint32_t)key;
+      if (a.idx64)
+        ((int64_t *)a.out_idx)[o] = (int64_t)id;
+      else
+        ((int32_t *)a.out_idx)[o] = id;
+      if (a.out_d) {
+        const uint32_t db = (uint32_t)(key >> 32);
+        float d = __uint_as_float(db);
+        if (db == kInfBits)
+          d = a.pad_dist;
+        else if (a.dist_mode == FK_DIST_SQRT)
+          d = sqrtf(d);
+        a.out_d[o] = d;
       }
     }
-    // count outside any lane-dependent branch: a ballot taken inside
-    // `if (lane == 0)` would see lane 0 alone
-    const int nexp = __popcll(__ballot(active));
-    if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
-    if (active) {
-      const int row = __float_as_int(qr.w);
-      write_row<KC>(a, t, out_base(a, ci, qc.w, row, k));
-    }
   }
+  if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
 }
 
 // ----------------------------------------------------- batched persistent
@@ -594,6 +760,80 @@ __global__ void k_candidate_count(const QueryArgs a, unsigned long long *total) 
   if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(total, c);
 }
 
+// Candidate count |C(q)| of every query, written to its output row.
+__global__ void k_point_candidates(const QueryArgs a, int32_t *out) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  if (qi >= a.nq || a.scalars[kStatus] != 0) return;
+  const int4 qc = a.qcell[qi];
+  const CloudInfo ci = a.clouds[qc.w];
+  long long c = 0;
+  for (int dx = -a.R; dx <= a.R; ++dx)
+    for (int dy = -a.R; dy <= a.R; ++dy) {
+      int b, e;
+      column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
+                   qc.z + a.R, b, e);
+      c += e - b;
+    }
+  out[__float_as_int(a.qrec[qi].w)] = (int32_t)(c < INT_MAX ? c : INT_MAX);
+}
+
+// Per-cloud candidate sums (self queries, block half-width a.R).
+__global__ void k_cand_per_cloud(const QueryArgs a, unsigned long long *acc) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = qi < a.nq && a.scalars[kStatus] == 0;
+  unsigned long long c = 0;
+  int cloud = 0;
+  if (ok) {
+    const int4 qc = a.qcell[qi];
+    cloud = qc.w;
+    const CloudInfo ci = a.clouds[cloud];
+    for (int dx = -a.R; dx <= a.R; ++dx)
+      for (int dy = -a.R; dy <= a.R; ++dy) {
+        int b, e;
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
+                     qc.z + a.R, b, e);
+        c += (unsigned long long)(e - b);
+      }
+  }
+  const uint64_t m = __ballot(ok);
+  if (m == 0ull) return;
+  const int c0 = __shfl(cloud, (int)__builtin_ctzll(m), kWave);
+  if (__ballot(ok && cloud != c0) == 0ull) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(acc + c0, c);
+  } else if (ok && c) {
+    atomicAdd(acc + cloud, c);
+  }
+}
+
+// side *= clamp((target_ratio * k / mean candidates)^(1/exponent)), bounded
+// so that the rebuilt cell box stays below ~64 cells per point.
+__global__ void k_adapt(const CloudInfo *clouds, int32_t B,
+                        const unsigned long long *acc, const int32_t *kpc,
+                        int32_t k, float target_ratio, float exponent,
+                        float *side, float *inv) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= B) return;
+  const CloudInfo ci = clouds[c];
+  const double n = (double)ci.num_points;
+  if (n <= 0.0 || acc[c] == 0ull) return;
+  const double mean = (double)acc[c] / n;
+  const double target = (double)target_ratio * (double)(kpc ? kpc[c] : k);
+  double f = pow(target / mean, 1.0 / (double)exponent);
+  f = fmin(fmax(f, 0.25), 4.0);
+  const double cells = (double)ci.nx * (double)ci.ny * (double)ci.nz;
+  f = fmax(f, cbrt(cells / (64.0 * n)));
+  const float s = (float)((double)side[c] * f);
+  side[c] = s;
+  inv[c] = (float)(1.0 / (double)s);
+}
+
+__global__ void k_zero_u64(unsigned long long *p, int32_t n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
 __global__ void k_reset_query_scalars(int32_t *scalars) {
   if (threadIdx.x == 0) {
     scalars[kPending] = 0;
@@ -603,6 +843,13 @@ __global__ void k_reset_query_scalars(int32_t *scalars) {
   }
 }
 
+// Fixed grid (graph-capturable): pending queries are read on the device.
+unsigned expand_blocks(int nq) {
+  const int waves = nq < 4096 ? nq : 4096;
+  const int b = (waves + kCoopWaves - 1) / kCoopWaves;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
 template <int KC>
 hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
   constexpr int WPB = waves_per_block<KC>();
@@ -610,10 +857,9 @@ hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
   const unsigned nb = (unsigned)((waves + WPB - 1) / WPB);
   if (a.passes & 2)
     hipLaunchKernelGGL(k_query<KC>, dim3(nb), dim3(64 * WPB), 0, s, a);
-  if (a.exact && (a.passes & 4)) {
-    unsigned ne = nb < 2048 ? nb : 2048;
-    hipLaunchKernelGGL(k_query_expand<KC>, dim3(ne), dim3(64 * WPB), 0, s, a);
-  }
+  if (a.exact && (a.passes & 4))
+    hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)),
+                       dim3(64 * kCoopWaves), 0, s, a);
   return hipGetLastError();
 }
 
@@ -629,10 +875,9 @@ hipError_t launch_batched_t(const QueryArgs &a, hipStream_t s) {
   const int per_cu = MAXKC <= 16 ? 4 : 5;
   unsigned nb = (unsigned)(cus * per_cu);
   hipLaunchKernelGGL(k_batched<MAXKC>, dim3(nb), dim3(64 * WPB), 0, s, a);
-  if (a.exact) {
-    hipLaunchKernelGGL(k_query_expand<MAXKC>, dim3(cus * 2), dim3(64 * WPB), 0, s,
-                       a);
-  }
+  if (a.exact)
+    hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)),
+                       dim3(64 * kCoopWaves), 0, s, a);
   return hipGetLastError();
 }
 
@@ -657,6 +902,28 @@ hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
   if (a.nq <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_candidate_count, dim3((a.nq + 255) / 256), dim3(256), 0, s,
                      a, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,
+                                   hipStream_t s) {
+  if (a.nq <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_point_candidates, dim3((a.nq + 255) / 256), dim3(256), 0,
+                     s, a, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *acc,
+                             const int32_t *k_per_cloud, int32_t k,
+                             float target_ratio, float exponent, float *side_io,
+                             float *inv_io, hipStream_t s) {
+  const int B = a.num_clouds;
+  hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
+  if (a.nq > 0)
+    hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + 255) / 256), dim3(256), 0, s,
+                       a, acc);
+  hipLaunchKernelGGL(k_adapt, dim3((B + 255) / 256), dim3(256), 0, s, a.clouds, B,
+                     acc, k_per_cloud, k, target_ratio, exponent, side_io, inv_io);
   return hipGetLastError();
 }
 

@@ -99,6 +99,38 @@ class Grid:
         self.num_points, self.num_clouds, self.num_queries = x.shape[0], int(o.shape[0]), -1
         return self
 
+    def auto_cell(self, xyz, offsets, k, scale):
+        """(side, inv) float32 [num_clouds] from the bounding-box density.
+        ``k`` is an int or an int32 device tensor of per-cloud k."""
+        x = _dev(xyz, torch.float32, "xyz")
+        o = _dev(offsets, torch.int32, "offsets").reshape(-1)
+        B = o.shape[0]
+        side = torch.empty(B, dtype=torch.float32, device=x.device)
+        inv = torch.empty(B, dtype=torch.float32, device=x.device)
+        kt = None
+        if isinstance(k, torch.Tensor):
+            kt = _dev(k, torch.int32, "k per cloud").reshape(-1)
+            kk = 0
+        else:
+            kk = int(k)
+        F.check(self._L.fk_grid_auto_cell(self._h, F.ptr(x), F.ptr(o), B, x.shape[0],
+                                          F.ptr(kt), kk, float(scale), F.ptr(side),
+                                          F.ptr(inv), F.stream_ptr()), "fk_grid_auto_cell")
+        return side, inv
+
+    def adapt_cell(self, k, side, inv, target_ratio=6.0, exponent=2.0):
+        """Density feedback on the current xyz build (in place on side/inv)."""
+        kt = None
+        if isinstance(k, torch.Tensor):
+            kt = _dev(k, torch.int32, "k per cloud").reshape(-1)
+            kk = 0
+        else:
+            kk = int(k)
+        F.check(self._L.fk_grid_adapt_cell(self._h, F.ptr(kt), kk, float(target_ratio),
+                                           float(exponent), F.ptr(side), F.ptr(inv),
+                                           F.stream_ptr()), "fk_grid_adapt_cell")
+        return side, inv
+
     def attach_queries(self, qxyz, qcell=None, qbatch=None, qoffsets=None, qpad=None,
                        no_sync=False):
         q = _dev(qxyz, torch.float32, "query xyz")
@@ -171,6 +203,17 @@ class Grid:
             F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
             ctypes.byref(t), F.stream_ptr()), "fk_candidate_count")
         return t.value
+
+    def point_candidates(self, radius_cells=1, attached=False):
+        """int32 [nq]: |C(q)| of every query (row order of the query set)."""
+        nq = self.num_queries if attached else self.num_points
+        out = torch.empty(max(nq, 0), dtype=torch.int32,
+                          device=torch.device("cuda", torch.cuda.current_device()))
+        F.check(self._L.fk_point_candidates(
+            self._h, int(radius_cells),
+            F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
+            F.ptr(out), F.stream_ptr()), "fk_point_candidates")
+        return out
 
     def batched_query(self, k_per_cloud, out_offsets, out_idx, out_dist, *, max_k,
                       mode=F.FK_MODE_EXACT, radius_cells=1, idx_dtype=torch.int64,

@@ -5,8 +5,12 @@ k nearest support points of every query within its cloud (fp32 squared
 distance ((dx*dx + dy*dy) + dz*dz), ties to the smaller index), the result
 pointops.knn_query computes by scanning whole segments
 (Pointcept/libs/pointops/src/knn_query/knn_query_cuda_kernel.cu:60-104).
-The grid's cell size per cloud comes from ``cell_sizes`` (autotuner table or
-density heuristic); exactness never depends on it, only speed does.
+
+Cell size: a first guess from each cloud's bounding-box density
+(fk_grid_auto_cell), then one density-feedback rebuild (fk_grid_adapt_cell)
+that moves the mean 27-cell candidate count towards ``target_ratio * k``.
+Exactness never depends on the cell size, only speed does; the target
+ratio comes from the autotuner table.
 """
 from __future__ import annotations
 
@@ -17,8 +21,13 @@ import torch
 from . import _lib as F
 from .grid import Grid
 
-# cell side = scale * (k * bbox_volume / n)^(1/3); tuned per (n, k, shape)
-DEFAULT_CELL_SCALE = 0.6
+# first guess: cell side = scale * (k * bbox_volume / n)^(1/3)
+DEFAULT_CELL_SCALE = 1.0
+# feedback: aim at target_ratio * k candidates per query in the 27 cells
+# (9: measured best for the 100k room at k = 16 and 32 on MI355X)
+DEFAULT_TARGET_RATIO = 9.0
+CANDIDATE_EXPONENT = 2.0  # candidates ~ side^2 on scanned surfaces
+ADAPT_ROUNDS = 3
 
 
 def cloud_ids(offsets: torch.Tensor, n: int) -> torch.Tensor:
@@ -27,14 +36,8 @@ def cloud_ids(offsets: torch.Tensor, n: int) -> torch.Tensor:
     return torch.searchsorted(offsets.contiguous(), ar, right=True)
 
 
-def cell_sizes(xyz: torch.Tensor, offsets: torch.Tensor, k, scale=None) -> torch.Tensor:
-    """Per-cloud cell side from the bounding-box density, on the device.
-
-    ``k`` is an int or a per-cloud tensor; ``scale`` an override of the
-    autotuned factor.  Returns float32 [num_clouds] (no host sync)."""
-    if scale is None:
-        from .autotune import lookup_scale
-        scale = lookup_scale(xyz.shape[0], k if isinstance(k, int) else int(k.max()))
+def _cell_sizes_torch(xyz, offsets, k, scale):
+    """Host restatement of fk_grid_auto_cell for CPU tensors."""
     B = offsets.shape[0]
     cid = cloud_ids(offsets, xyz.shape[0]).long()
     big = torch.full((B, 3), float("inf"), device=xyz.device)
@@ -43,17 +46,47 @@ def cell_sizes(xyz: torch.Tensor, offsets: torch.Tensor, k, scale=None) -> torch
     ends = offsets.long()
     starts = torch.cat([ends.new_zeros(1), ends[:-1]])
     n = (ends - starts).clamp_min(1).double()
-    ext = (mx - mn).double().clamp_min(0)
+    ext = (mx.double() - mn.double()).clamp_min(0)
     ext = torch.where(torch.isfinite(ext), ext, torch.zeros_like(ext))
     span = ext.max(1).values.clamp_min(1e-6)
-    # degenerate axes (planar / linear clouds) count with 1% of the span
     vol = ext.clamp_min(1e-2 * span[:, None]).prod(1)
     kk = torch.as_tensor(k, device=xyz.device).double().expand(B)
     side = float(scale) * (kk * vol / n).pow(1.0 / 3.0)
-    # keep dense boxes bounded: at most ~64 cells per point per cloud
-    min_side = (vol / (64.0 * n)).pow(1.0 / 3.0)
-    side = torch.maximum(side, min_side).clamp_min(1e-6)
+    side = torch.maximum(side, (vol / (64.0 * n)).pow(1.0 / 3.0)).clamp_min(1e-6)
     return side.float()
+
+
+_util = {}
+
+
+def cell_sizes(xyz: torch.Tensor, offsets: torch.Tensor, k, scale=None) -> torch.Tensor:
+    """Per-cloud first-guess cell side from the bounding-box density.
+
+    On a HIP device this runs fk_grid_auto_cell (no host sync); CPU tensors
+    use the identical torch formula."""
+    if scale is None:
+        scale = DEFAULT_CELL_SCALE
+    if not xyz.is_cuda:
+        return _cell_sizes_torch(xyz.float(), offsets, k, scale)
+    dev = torch.cuda.current_device()
+    g = _util.get(dev)
+    if g is None:
+        g = _util[dev] = Grid()
+    side, _ = g.auto_cell(xyz, offsets, k, scale)
+    return side
+
+
+def adapt_rounds(g: Grid, xyz, offsets, k, side, inv, target_ratio, rounds=ADAPT_ROUNDS):
+    """Density feedback: each round moves every cloud's cell side by at most
+    4x towards ``target_ratio * k`` candidates per query and rebuilds; stops
+    early once no cloud moves by more than 20 %."""
+    for _ in range(rounds):
+        before = side.clone()
+        g.adapt_cell(k, side, inv, target_ratio, CANDIDATE_EXPONENT)
+        g.build_xyz(xyz, offsets, inv)
+        change = (side.double() / before.double()).log().abs().max()
+        if float(change) < math.log(1.2):
+            break
 
 
 class ExactKNN:
@@ -62,26 +95,47 @@ class ExactKNN:
     def __init__(self):
         self.grid = Grid()
         self.last_stats = None
+        self.side = None
+
+    def build(self, xyz, offsets, k, *, cell=None, scale=None, target_ratio=None,
+              adapt=True):
+        """Build the support grid; returns the per-cloud cell side."""
+        F.require_device(xyz, "xyz")
+        xyz = xyz.float().contiguous()
+        offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
+        g = self.grid
+        if cell is not None:
+            side = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
+            inv = (1.0 / side.double()).float()
+            g.build_xyz(xyz, offsets, inv)
+        else:
+            kmax = k if isinstance(k, int) else int(k.max())
+            if target_ratio is None:
+                from .autotune import lookup_target_ratio
+                target_ratio = lookup_target_ratio(xyz.shape[0], kmax)
+            side, inv = g.auto_cell(xyz, offsets, k,
+                                    DEFAULT_CELL_SCALE if scale is None else scale)
+            g.build_xyz(xyz, offsets, inv)
+            if adapt:
+                adapt_rounds(g, xyz, offsets, k, side, inv, target_ratio)
+        self.side = side
+        return side
 
     @torch.no_grad()
     def __call__(self, xyz, offsets, k, new_xyz=None, new_offsets=None, *,
                  idx_dtype=torch.int64, pad_mode=F.FK_PAD_MINUS1,
                  dist_mode=F.FK_DIST_SQ, pad_dist=math.inf, cell=None, scale=None,
-                 cut_r2=math.inf, stats=False):
+                 target_ratio=None, adapt=True, cut_r2=math.inf, stats=False):
         if k < 1 or k > F.FK_MAX_K:
             raise ValueError(f"k must be in [1, {F.FK_MAX_K}], got {k}")
-        F.require_device(xyz, "xyz")
-        xyz = xyz.float().contiguous()
-        offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
-        if cell is None:
-            cell = cell_sizes(xyz, offsets, k, scale)
-        inv = (1.0 / cell.double()).float().to(xyz.device)
+        self.build(xyz, offsets, k, cell=cell, scale=scale, target_ratio=target_ratio,
+                   adapt=adapt)
         g = self.grid
-        g.build_xyz(xyz, offsets, inv)
         attached = new_xyz is not None
         if attached:
             if new_offsets is None:
                 raise ValueError("new_offsets required with new_xyz")
+            F.require_device(new_xyz, "new_xyz")
             g.attach_queries(new_xyz.float().contiguous(),
                              qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32))
         if pad_mode == F.FK_PAD_SELF and attached:

@@ -154,6 +154,25 @@ int fk_grid_build_xyz(fk_grid *grid, const float *xyz, const int32_t *offsets,
                       int32_t num_clouds, int64_t n, const float *inv_cell,
                       int32_t flags, void *stream);
 
+/* Cell side per cloud for exact queries, from the cloud's bounding-box
+ * density: side = scale * (k * volume / n)^(1/3) (degenerate axes count with
+ * 1% of the largest extent; at most ~64 cells per point).  k from
+ * k_per_cloud (int32[num_clouds], device) or the scalar k.  Writes
+ * side_out (nullable) and inv_out = 1/side (float[num_clouds], device); uses
+ * the handle's per-cloud scratch; no host synchronisation.  No reference
+ * counterpart (the reference's caller picks voxel size and num_down). */
+int fk_grid_auto_cell(fk_grid *grid, const float *xyz, const int32_t *offsets,
+                      int32_t num_clouds, int64_t n, const int32_t *k_per_cloud,
+                      int32_t k, float scale, float *side_out, float *inv_out,
+                      void *stream);
+/* Density feedback on a grid built by fk_grid_build_xyz: per cloud, the mean
+ * 27-cell candidate count m of the self queries gives
+ * side *= clamp((target_ratio * k / m)^(1/exponent), 1/4, 4) and
+ * inv = 1/side, in place (device); rebuild with the new inv afterwards. */
+int fk_grid_adapt_cell(fk_grid *grid, const int32_t *k_per_cloud, int32_t k,
+                       float target_ratio, float exponent, float *side_io,
+                       float *inv_io, void *stream);
+
 /* Attach a query list in the grid's cell space.  For grids built from
  * cells: qcell (int64[nq,3], device) are voxel coordinates (shifted by the
  * grid's num_down) and qbatch (int64[nq], device, nullable) the cloud.  For
@@ -194,6 +213,11 @@ int fk_query_stats(fk_grid *grid, int64_t *first_pass_certified,
  * FlashKnnWrapper.py:351-352, is this sum / N).  Synchronises. */
 int fk_candidate_count(fk_grid *grid, int32_t radius_cells, int32_t query_set,
                        int64_t *total, void *stream);
+/* Per-query candidate count |C(q)| (int32[nq], device, row order of the
+ * query set); the node statistic the reference's adaptive level selection
+ * thresholds (FlashKNN/functions/adaptive_octree.py:204-208,312-351). */
+int fk_point_candidates(fk_grid *grid, int32_t radius_cells, int32_t query_set,
+                        int32_t *out, void *stream);
 
 /* -------------------------------------- batched persistent query (C4) */
 /* One persistent launch over all clouds of a grid built by

@@ -284,3 +284,60 @@ def pointops_interpolation_weights(dist):
     normalised (Pointcept/libs/pointops/functions/interpolation.py:15-17)."""
     r = 1.0 / (np.asarray(dist, np.float64) + 1e-8)
     return r / r.sum(1, keepdims=True)
+
+
+# ------------------------------------------- adaptive per-point levels
+
+def point_candidates(cell, batch=None, R=1):
+    """|C(q)| for every point: points of the same batch whose cell is within
+    R per axis (numpy: unique cells, searchsorted of the (2R+1)^3 offsets)."""
+    c = _i64(cell).reshape(-1, 3)
+    b = np.zeros(len(c), np.int64) if batch is None else _i64(batch).reshape(-1)
+    c = c - c.min(0) + R
+    W = c.max(0) + R + 1
+    key = ((b * W[0] + c[:, 0]) * W[1] + c[:, 1]) * W[2] + c[:, 2]
+    uk, cnt = np.unique(key, return_counts=True)
+    total = np.zeros(len(c), np.int64)
+    for dx in range(-R, R + 1):
+        for dy in range(-R, R + 1):
+            for dz in range(-R, R + 1):
+                nk = ((b * W[0] + c[:, 0] + dx) * W[1] + c[:, 1] + dy) * W[2] + c[:, 2] + dz
+                pos = np.searchsorted(uk, nk)
+                ok = pos < len(uk)
+                hit = np.zeros(len(c), bool)
+                hit[ok] = uk[pos[ok]] == nk[ok]
+                total += np.where(hit, cnt[np.minimum(pos, len(uk) - 1)], 0)
+    return total
+
+
+def adaptive_query(grid_coord, batch_idx, xyz, k, min_f=2, max_f=8, max_shift=None,
+                   cut_radius=float("inf")):
+    """AdaptiveNeighborhoodFlashKNN.query
+    (FlashKNN/functions/adaptive_octree.py:129-146 shift per batch,
+    :204-208 27-node counts, :312-351 level selection, then the 27-node
+    query at the selected level).  Returns (idx, d, selected level)."""
+    g = _i64(grid_coord).reshape(-1, 3)
+    b = np.zeros(len(g), np.int64) if batch_idx is None else _i64(batch_idx)
+    B = int(b.max()) + 1
+    mins = np.full((B, 3), np.iinfo(np.int64).max, np.int64)
+    np.minimum.at(mins, b, g)
+    g = g - mins[b]
+    if max_shift is None:
+        max_shift = int(g.max()).bit_length()
+    counts = [point_candidates(g >> s, b) for s in range(max_shift + 1)]
+    sel = np.full(len(g), max_shift, np.int64)
+    active = counts[max_shift] > max_f * k
+    for s in range(max_shift - 1, -1, -1):
+        use = active & (counts[s] >= min_f * k)
+        sel[use] = s
+        active = use & (counts[s] > max_f * k)
+    pts = g.astype(np.float32) if xyz is None else _f32(xyz)
+    idx = np.empty((len(g), k), np.int64)
+    d = np.empty((len(g), k), np.float32)
+    for s in np.unique(sel):
+        q = np.where(sel == s)[0]
+        cell = g >> s
+        qi, qd = grid_knn(pts, cell, b, pts[q], cell[q], b[q], q, k, 1,
+                          cut_radius * cut_radius)
+        idx[q], d[q] = qi, qd
+    return idx, d, sel

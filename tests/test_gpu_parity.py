@@ -179,8 +179,8 @@ def test_exact_uniform_vs_brute(oracle, k):
     _same(idx, d, ref_i, ref_d)
 
 
-@pytest.mark.parametrize("scale", [0.3, 0.6, 1.5])
-def test_exact_room_lidar_any_cell(oracle, scale):
+@pytest.mark.parametrize("cell", [0.03, 0.15, 0.8, None])
+def test_exact_room_lidar_any_cell(oracle, cell):
     from flashknn_exprepo_amd import synthetic
     rng = np.random.default_rng(9)
     a = synthetic.subsample_to(synthetic.room(seed=48, target=30000), 12000, rng)
@@ -188,7 +188,8 @@ def test_exact_room_lidar_any_cell(oracle, scale):
     xyz, off = synthetic.pack([a, b])
     ref_i, ref_d = oracle.brute_knn(xyz, off, xyz, off, 16)
     eng = fk.ExactKNN()
-    idx, d = eng(_t(xyz), _t(off), 16, scale=scale, stats=True)
+    c = None if cell is None else torch.tensor([cell, cell * 2], device=_dev())
+    idx, d = eng(_t(xyz), _t(off), 16, cell=c, stats=True)
     _same(idx, d, ref_i, ref_d)
     st = eng.last_stats
     assert st["certified_first_pass"] + st["expanded"] == len(xyz)
@@ -433,3 +434,66 @@ def test_grid_tables_partition(oracle):
     counts = np.bincount(lin, minlength=1000)
     assert np.array_equal(np.diff(cs), counts)
     assert np.array_equal(np.sort(lin[ids]), lin[ids])  # records are cell-sorted
+
+
+def test_auto_cell_matches_host_formula():
+    from flashknn_exprepo_amd.knn import _cell_sizes_torch, cell_sizes
+    rng = np.random.default_rng(22)
+    a = rng.random((3000, 3), dtype=np.float32)
+    b = (rng.random((5000, 3), dtype=np.float32) * [10, 10, 0.001]).astype(np.float32)
+    xyz, off = np.concatenate([a, b]), np.array([3000, 8000], np.int32)
+    gpu = cell_sizes(_t(xyz), _t(off), 16, scale=1.0).cpu()
+    cpu = _cell_sizes_torch(torch.from_numpy(xyz), torch.from_numpy(off), 16, 1.0)
+    torch.testing.assert_close(gpu, cpu, rtol=1e-5, atol=0)
+
+
+def test_exact_far_queries_expand_to_cover(oracle):
+    rng = np.random.default_rng(23)
+    s = rng.random((2000, 3), dtype=np.float32)
+    q = np.array([[50, 50, 50], [-30, 0.5, 0.5], [0.5, 0.5, 0.5], [1.2, -0.3, 0.7]],
+                 np.float32)
+    ref_i, ref_d = oracle.brute_knn(s, [2000], q, [4], 16)
+    eng = fk.ExactKNN()
+    idx, d = eng(_t(s), _t(np.array([2000], np.int32)), 16, _t(q),
+                 _t(np.array([4], np.int32)), stats=True)
+    _same(idx, d, ref_i, ref_d)
+    assert eng.last_stats["expanded"] >= 2
+
+
+def test_exact_sparse_clusters(oracle):
+    """Dense blobs far apart: most blocks hold fewer than k points, so the
+    expansion has to double its radius before it can certify."""
+    rng = np.random.default_rng(24)
+    blobs = [rng.normal(c, 0.01, (20, 3)) for c in rng.uniform(-20, 20, (40, 3))]
+    s = np.concatenate(blobs).astype(np.float32)
+    ref_i, ref_d = oracle.brute_knn(s, [len(s)], s, [len(s)], 32)
+    idx, d = fk.knn(_t(s), _t(np.array([len(s)], np.int32)), 32, cell=torch.tensor([0.5]))
+    _same(idx, d, ref_i, ref_d)
+
+
+@pytest.mark.parametrize("k", [8, 16])
+def test_adaptive_neighborhood_lidar(oracle, k):
+    from flashknn_exprepo_amd import synthetic
+    rng = np.random.default_rng(25)
+    pts = synthetic.subsample_to(synthetic.lidar(seed=49), 20000, rng)
+    grid = np.floor(pts / np.float32(0.05)).astype(np.int64)
+    batch = np.zeros(len(pts), np.int64)
+    ref_i, ref_d, ref_sel = oracle.adaptive_query(grid, batch, pts, k)
+    a = fk.AdaptiveNeighborhoodFlashKNN(num_nbr=k)
+    idx, d = a.query(_t(grid), _t(batch), _t(pts), return_distances=True)
+    _same(idx, d, ref_i, ref_d)
+    assert sum(s["query_points"] for s in a.last_stats["selected_levels"]) == len(pts)
+
+
+def test_adaptive_neighborhood_batches_no_xyz(oracle):
+    rng = np.random.default_rng(26)
+    g1 = rng.integers(0, 30, (3000, 3))
+    g2 = rng.integers(100, 400, (2000, 3))
+    grid = np.concatenate([g1, g2]).astype(np.int64)
+    batch = np.concatenate([np.zeros(3000), np.ones(2000)]).astype(np.int64)
+    ref_i, ref_d, _ = oracle.adaptive_query(grid, batch, None, 12)
+    a = fk.AdaptiveNeighborhoodFlashKNN(num_nbr=12)
+    idx, d = a.query(_t(grid), _t(batch), None, return_distances=True)
+    _same(idx, d, ref_i, ref_d)
+    sel = fk.select_adaptive_levels(fk.build_adaptive_octree(_t(grid), _t(batch)), 12)
+    assert sel.dtype == torch.int16

@@ -259,3 +259,32 @@ def test_idw_weights_rows_sum_to_one(oracle):
     w = oracle.idw_weights(d2, 1e-8)
     np.testing.assert_allclose(w.sum(1), 1.0)
     assert w[0, 0] > 0.999  # d2 = 0 dominates but stays finite
+
+
+def test_point_candidates_vs_bruteforce(oracle):
+    rng = np.random.default_rng(30)
+    cell = rng.integers(-3, 4, (300, 3))
+    batch = rng.integers(0, 2, 300)
+    got = oracle.point_candidates(cell, batch)
+    d = np.abs(cell[:, None, :] - cell[None, :, :]).max(-1)
+    same = batch[:, None] == batch[None, :]
+    assert np.array_equal(got, ((d <= 1) & same).sum(1))
+
+
+def test_adaptive_selection_rule(oracle):
+    # a dense blob and a sparse line: dense points descend to fine levels,
+    # sparse ones stay coarse; every point's candidate set at its level is
+    # at least min_f * k unless it sits at the coarsest level
+    rng = np.random.default_rng(31)
+    dense = rng.integers(0, 8, (400, 3))
+    sparse = np.stack([np.arange(0, 600, 20), np.zeros(30, int), np.zeros(30, int)], 1)
+    g = np.concatenate([dense, sparse]).astype(np.int64)
+    idx, d, sel = oracle.adaptive_query(g, None, None, 8)
+    top = int(g.max()).bit_length()
+    assert sel[:400].mean() < sel[400:].mean()
+    for s in np.unique(sel):
+        q = np.where(sel == s)[0]
+        cnt = oracle.point_candidates(g >> s, None)[q]
+        assert (s == top) or (cnt >= 16).all()
+    # rows are sorted and padded with the point itself
+    assert (np.diff(d, axis=1) >= 0).all()
