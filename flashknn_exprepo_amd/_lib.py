@@ -97,8 +97,9 @@ SIGNATURES = [
     ("fk_grid_attach_queries", ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _i64, _vp, _i32, _vp]),
     ("fk_grid_auto_cell", ctypes.c_int, [_vp, _vp, _vp, _i32, _i64, _vp, _i32, ctypes.c_float,
                                          _vp, _vp, _vp]),
-    ("fk_grid_adapt_cell", ctypes.c_int, [_vp, _vp, _i32, ctypes.c_float, ctypes.c_float,
-                                          _vp, _vp, _vp]),
+    ("fk_grid_adapt_cell", ctypes.c_int, [_vp, _vp, _i32, ctypes.c_float, _vp,
+                                          ctypes.c_float, _vp, _vp, _vp]),
+    ("fk_grid_cloud_candidates", ctypes.c_int, [_vp, _vp, _vp]),
     ("fk_grid_status", ctypes.c_int, [_vp, _vp]),
     ("fk_grid_info", ctypes.c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i32),
                                     ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),

@@ -24,15 +24,15 @@ _TABLE = None
 TABLE_PATH = os.environ.get(
     "FLASHKNN_TUNE_TABLE",
     os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_table.json"))
-RATIOS = (3.0, 4.5, 6.0, 8.0, 11.0)
+RATIOS = (4.5, 6.5, 9.0, 13.0, 19.0, 28.0)
 
 
 def _kclass(k: int) -> int:
     return 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 64
 
 
-def bucket(n: int, k: int) -> str:
-    return f"n{max(0, int(math.log2(max(n, 1))))}_k{_kclass(k)}"
+def bucket(n: int, k: int, dclass: int = 0) -> str:
+    return f"n{max(0, int(math.log2(max(n, 1))))}_k{_kclass(k)}_d{int(dclass)}"
 
 
 def _load():
@@ -46,12 +46,22 @@ def _load():
     return _TABLE
 
 
-def lookup_target_ratio(n: int, k: int) -> float:
+def lookup_target_ratio(n: int, k: int, dclass: int = 0) -> float:
+    """Tuned ratio of the bucket; else the nearest tuned size with the same
+    k class and density class; else the default."""
     from .knn import DEFAULT_TARGET_RATIO
     with _LOCK:
         t = _load()
-    e = t.get(bucket(n, k))
-    return float(e["target_ratio"]) if e else DEFAULT_TARGET_RATIO
+    e = t.get(bucket(n, k, dclass))
+    if e:
+        return float(e["target_ratio"])
+    suffix = f"_k{_kclass(k)}_d{int(dclass)}"
+    ln = max(0, int(math.log2(max(n, 1))))
+    near = [(abs(int(key[1:key.index("_")]) - ln), v) for key, v in t.items()
+            if key.endswith(suffix)]
+    if near:
+        return float(min(near, key=lambda x: x[0])[1]["target_ratio"])
+    return DEFAULT_TARGET_RATIO
 
 
 def _time_ms(fn, reps=5):
@@ -67,9 +77,13 @@ def _time_ms(fn, reps=5):
 
 
 def tune(xyz, offsets, k, ratios=RATIOS, save=True):
-    """Time the exact query at each target ratio; record and return the best."""
-    from .knn import ExactKNN
+    """Time the exact query of one cloud at each target ratio; record the
+    fastest under the cloud's (size, k, density class) bucket."""
+    from .knn import ExactKNN, density_class
     eng = ExactKNN()
+    eng.build(xyz, offsets, k, adapt=False)
+    n = xyz.shape[0]
+    dclass = density_class(float(eng.grid.cloud_candidates()[0]) / max(n, 1), k)
     results = {}
     for r in ratios:
         eng.build(xyz, offsets, k, target_ratio=r)
@@ -78,9 +92,45 @@ def tune(xyz, offsets, k, ratios=RATIOS, save=True):
     if save:
         with _LOCK:
             t = _load()
-            t[bucket(xyz.shape[0], k)] = {"target_ratio": best, "ms": results}
+            t[bucket(n, k, dclass)] = {"target_ratio": best, "ms": results,
+                                       "density_class": dclass}
             tmp = TABLE_PATH + ".tmp"
             with open(tmp, "w") as f:
                 json.dump(t, f, indent=1, sort_keys=True)
             os.replace(tmp, TABLE_PATH)
-    return best, results
+    return best, results, dclass
+
+
+def main(argv=None):
+    """Tune the synthetic room / LiDAR workloads at a few sizes and k and
+    write the table: ``python -m flashknn_exprepo_amd.autotune --out PATH``."""
+    import argparse
+
+    import numpy as np
+
+    from . import synthetic
+
+    global TABLE_PATH, _TABLE
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--sizes", default="100000,30000,10000")
+    ap.add_argument("--ks", default="8,16,32")
+    args = ap.parse_args(argv)
+    TABLE_PATH, _TABLE = args.out, {}
+    dev = torch.device("cuda", 0)
+    bases = {"room": synthetic.room(seed=48), "lidar": synthetic.lidar(seed=49)}
+    for name, base in bases.items():
+        for n in (int(s) for s in args.sizes.split(",")):
+            pts = synthetic.subsample_to(base, n, np.random.default_rng(n))
+            x = torch.from_numpy(pts).to(dev)
+            o = torch.tensor([len(pts)], dtype=torch.int32, device=dev)
+            for k in (int(s) for s in args.ks.split(",")):
+                best, res, dc = tune(x, o, k)
+                print(f"{name} n={len(pts)} k={k} d={dc}: best ratio {best} "
+                      f"({res[best]:.4f} ms; " +
+                      ", ".join(f"{r}:{m:.4f}" for r, m in sorted(res.items())) + ")",
+                      flush=True)
+
+
+if __name__ == "__main__":
+    main()

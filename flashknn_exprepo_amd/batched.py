@@ -16,8 +16,7 @@ import torch
 
 from . import _lib as F
 from .grid import Grid
-from .autotune import lookup_target_ratio
-from .knn import DEFAULT_CELL_SCALE, adapt_rounds
+from .knn import build_exact_grid
 
 
 class BatchedKNN:
@@ -47,20 +46,10 @@ class BatchedKNN:
         self.out_idx = torch.empty(total, dtype=idx_dtype, device=xyz.device)
         self.out_dist = torch.empty(total, dtype=torch.float32, device=xyz.device)
         self.grid = Grid()
-        if cell is None:
-            # first guess from each cloud's density, one feedback rebuild
-            side, inv = self.grid.auto_cell(self.xyz, self.offsets, self.k,
-                                            DEFAULT_CELL_SCALE if scale is None else scale)
-            self.grid.build_xyz(self.xyz, self.offsets, inv)
-            if target_ratio is None:
-                target_ratio = lookup_target_ratio(self.xyz.shape[0], self.max_k)
-            adapt_rounds(self.grid, self.xyz, self.offsets, self.k, side, inv,
-                         target_ratio)
-            cell = side
-        else:
-            cell = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
-            self.grid.build_xyz(self.xyz, self.offsets, (1.0 / cell.double()).float())
-        self.cell = cell
+        # per-cloud first guess + density feedback (sizes read back: sync builds)
+        self.cell = build_exact_grid(self.grid, self.xyz, self.offsets, self.k, cell=cell,
+                                     scale=scale, target_ratio=target_ratio,
+                                     no_sync=False)
         self._captured = False
 
     def run(self):

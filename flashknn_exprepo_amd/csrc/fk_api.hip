@@ -456,6 +456,7 @@ int fk_grid_status(fk_grid *g, void *stream) {
   FK_CHECK(read_scalars(g, S(stream)));
   const int st = g->h_pinned[kStatus];
   if (st != 0) return status_message(st);
+  if (g->built) g->n_cells = g->h_pinned[kTotalCells];  // known after a no-sync build
   return ok();
 }
 
@@ -539,12 +540,8 @@ int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
   QueryArgs a;
   FK_CHECK(make_args(g, p, out_idx, out_dist, a));
   if (a.exact) {
-    if (g->reserved) {
-      if ((int64_t)a.nq > g->cap_pend || (int64_t)a.nq * a.k > g->cap_pkeys)
-        return fail(FK_ERR_CAPACITY, "fk_query: reserved pending capacity too small");
-    } else {
-      FK_CHECK(ensure_pending(g, a.nq, a.k));
-    }
+    // grows only when the reserve (or the last query) was smaller
+    FK_CHECK(ensure_pending(g, a.nq, a.k));
     a.pend_q = g->pend_q;
     a.pend_keys = g->pend_keys;
   }
@@ -624,14 +621,34 @@ int fk_grid_auto_cell(fk_grid *g, const float *xyz, const int32_t *offsets,
   return ok();
 }
 
+int fk_grid_cloud_candidates(fk_grid *g, int64_t *out, void *stream) {
+  FK_CHECK(check_stream_device(g));
+  if (!g->built) return fail(FK_ERR_STATE, "fk_grid_cloud_candidates: grid not built");
+  if (!out) return fail(FK_ERR_INVALID_ARG, "fk_grid_cloud_candidates: out is NULL");
+  fk_query_params p;
+  memset(&p, 0, sizeof(p));
+  p.k = 1;
+  p.mode = FK_MODE_CELLS;
+  p.radius_cells = 1;
+  p.cut_r2 = INFINITY;
+  p.query_set = FK_QUERIES_SELF;
+  QueryArgs a;
+  int dummy = 0;
+  FK_CHECK(make_args(g, &p, &dummy, nullptr, a));
+  a.num_clouds = g->n_clouds;
+  FK_HIP(launch_cloud_candidates(a, (unsigned long long *)out, S(stream)));
+  return ok();
+}
+
 int fk_grid_adapt_cell(fk_grid *g, const int32_t *k_per_cloud, int32_t k,
-                       float target_ratio, float exponent, float *side_io,
-                       float *inv_io, void *stream) {
+                       float target_ratio, const float *target_per_cloud,
+                       float exponent, float *side_io, float *inv_io,
+                       void *stream) {
   FK_CHECK(check_stream_device(g));
   if (!g->built || !g->from_xyz)
     return fail(FK_ERR_STATE, "fk_grid_adapt_cell needs a grid built from xyz");
-  if (!side_io || !inv_io || !(target_ratio > 0.f) || !(exponent > 0.f) ||
-      (!k_per_cloud && (k < 1 || k > FK_MAX_K)))
+  if (!side_io || !inv_io || (!target_per_cloud && !(target_ratio > 0.f)) ||
+      !(exponent > 0.f) || (!k_per_cloud && (k < 1 || k > FK_MAX_K)))
     return fail(FK_ERR_INVALID_ARG, "fk_grid_adapt_cell: bad arguments");
   fk_query_params p;
   memset(&p, 0, sizeof(p));
@@ -645,7 +662,7 @@ int fk_grid_adapt_cell(fk_grid *g, const int32_t *k_per_cloud, int32_t k,
   FK_CHECK(make_args(g, &p, &dummy, nullptr, a));
   a.num_clouds = g->n_clouds;
   FK_HIP(launch_adapt_cell(a, g->cloud_acc, k_per_cloud, k, target_ratio,
-                           exponent, side_io, inv_io, S(stream)));
+                           target_per_cloud, exponent, side_io, inv_io, S(stream)));
   return ok();
 }
 
@@ -662,11 +679,7 @@ static int batched_args(fk_grid *g, const fk_query_params *p,
     return fail(FK_ERR_INVALID_ARG, "fk_batched_query: k_per_cloud and "
                                     "out_offsets required");
   if (a.exact) {
-    if ((int64_t)a.nq > g->cap_pend || (int64_t)a.nq * a.k > g->cap_pkeys) {
-      if (g->reserved)
-        return fail(FK_ERR_CAPACITY, "reserved pending capacity too small");
-      FK_CHECK(ensure_pending(g, a.nq, a.k));
-    }
+    FK_CHECK(ensure_pending(g, a.nq, a.k));
     a.pend_q = g->pend_q;
     a.pend_keys = g->pend_keys;
   }

@@ -155,8 +155,11 @@ hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,
 // and inv = 1/side, in place on the device.
 hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *per_cloud,
                              const int32_t *k_per_cloud, int32_t k,
-                             float target_ratio, float exponent, float *side_io,
-                             float *inv_io, hipStream_t s);
+                             float target_ratio, const float *ratio_per_cloud,
+                             float exponent, float *side_io, float *inv_io,
+                             hipStream_t s);
+hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
+                                   hipStream_t s);
 hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
                         int32_t B, int32_t *prefix, int32_t *scalars,
                         hipStream_t s);

@@ -811,15 +811,16 @@ __global__ void k_cand_per_cloud(const QueryArgs a, unsigned long long *acc) {
 // so that the rebuilt cell box stays below ~64 cells per point.
 __global__ void k_adapt(const CloudInfo *clouds, int32_t B,
                         const unsigned long long *acc, const int32_t *kpc,
-                        int32_t k, float target_ratio, float exponent,
-                        float *side, float *inv) {
+                        int32_t k, float target_ratio, const float *ratio_pc,
+                        float exponent, float *side, float *inv) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= B) return;
   const CloudInfo ci = clouds[c];
   const double n = (double)ci.num_points;
   if (n <= 0.0 || acc[c] == 0ull) return;
   const double mean = (double)acc[c] / n;
-  const double target = (double)target_ratio * (double)(kpc ? kpc[c] : k);
+  const double ratio = ratio_pc ? (double)ratio_pc[c] : (double)target_ratio;
+  const double target = ratio * (double)(kpc ? kpc[c] : k);
   double f = pow(target / mean, 1.0 / (double)exponent);
   f = fmin(fmax(f, 0.25), 4.0);
   const double cells = (double)ci.nx * (double)ci.ny * (double)ci.nz;
@@ -913,17 +914,29 @@ hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,
   return hipGetLastError();
 }
 
+hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
+                                   hipStream_t s) {
+  const int B = a.num_clouds;
+  hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
+  if (a.nq > 0)
+    hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + 255) / 256), dim3(256), 0, s,
+                       a, acc);
+  return hipGetLastError();
+}
+
 hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *acc,
                              const int32_t *k_per_cloud, int32_t k,
-                             float target_ratio, float exponent, float *side_io,
-                             float *inv_io, hipStream_t s) {
+                             float target_ratio, const float *ratio_pc,
+                             float exponent, float *side_io, float *inv_io,
+                             hipStream_t s) {
   const int B = a.num_clouds;
   hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
   if (a.nq > 0)
     hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + 255) / 256), dim3(256), 0, s,
                        a, acc);
   hipLaunchKernelGGL(k_adapt, dim3((B + 255) / 256), dim3(256), 0, s, a.clouds, B,
-                     acc, k_per_cloud, k, target_ratio, exponent, side_io, inv_io);
+                     acc, k_per_cloud, k, target_ratio, ratio_pc, exponent, side_io,
+                     inv_io);
   return hipGetLastError();
 }
 

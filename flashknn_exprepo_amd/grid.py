@@ -118,18 +118,32 @@ class Grid:
                                           F.ptr(inv), F.stream_ptr()), "fk_grid_auto_cell")
         return side, inv
 
-    def adapt_cell(self, k, side, inv, target_ratio=6.0, exponent=2.0):
-        """Density feedback on the current xyz build (in place on side/inv)."""
+    def adapt_cell(self, k, side, inv, target_ratio=9.0, exponent=2.0):
+        """Density feedback on the current xyz build (in place on side/inv).
+        ``target_ratio``: a float, or a float32 device tensor per cloud."""
         kt = None
         if isinstance(k, torch.Tensor):
             kt = _dev(k, torch.int32, "k per cloud").reshape(-1)
             kk = 0
         else:
             kk = int(k)
-        F.check(self._L.fk_grid_adapt_cell(self._h, F.ptr(kt), kk, float(target_ratio),
+        rt, r = None, 1.0
+        if isinstance(target_ratio, torch.Tensor):
+            rt = _dev(target_ratio, torch.float32, "target ratios").reshape(-1)
+        else:
+            r = float(target_ratio)
+        F.check(self._L.fk_grid_adapt_cell(self._h, F.ptr(kt), kk, r, F.ptr(rt),
                                            float(exponent), F.ptr(side), F.ptr(inv),
                                            F.stream_ptr()), "fk_grid_adapt_cell")
         return side, inv
+
+    def cloud_candidates(self):
+        """int64 [num_clouds]: sum of 27-cell candidate counts of each cloud."""
+        out = torch.empty(self.num_clouds, dtype=torch.int64,
+                          device=torch.device("cuda", torch.cuda.current_device()))
+        F.check(self._L.fk_grid_cloud_candidates(self._h, F.ptr(out), F.stream_ptr()),
+                "fk_grid_cloud_candidates")
+        return out
 
     def attach_queries(self, qxyz, qcell=None, qbatch=None, qoffsets=None, qpad=None,
                        no_sync=False):

@@ -7,10 +7,13 @@ pointops.knn_query computes by scanning whole segments
 (Pointcept/libs/pointops/src/knn_query/knn_query_cuda_kernel.cu:60-104).
 
 Cell size: a first guess from each cloud's bounding-box density
-(fk_grid_auto_cell), then one density-feedback rebuild (fk_grid_adapt_cell)
-that moves the mean 27-cell candidate count towards ``target_ratio * k``.
+(fk_grid_auto_cell), then density-feedback rebuilds (fk_grid_adapt_cell)
+that move the mean 27-cell candidate count towards ``target_ratio * k``.
 Exactness never depends on the cell size, only speed does; the target
-ratio comes from the autotuner table.
+ratio comes from the autotuner table, keyed by cloud size, k and a density
+class measured at the first guess.  Builds use reserved capacity and no
+host synchronisation; one status read at the end of a call reports errors
+(and a cell box that outgrew the reserve is rebuilt with exact sizing).
 """
 from __future__ import annotations
 
@@ -27,7 +30,7 @@ DEFAULT_CELL_SCALE = 1.0
 # (9: measured best for the 100k room at k = 16 and 32 on MI355X)
 DEFAULT_TARGET_RATIO = 9.0
 CANDIDATE_EXPONENT = 2.0  # candidates ~ side^2 on scanned surfaces
-ADAPT_ROUNDS = 3
+ADAPT_ROUNDS = 2
 
 
 def cloud_ids(offsets: torch.Tensor, n: int) -> torch.Tensor:
@@ -76,17 +79,49 @@ def cell_sizes(xyz: torch.Tensor, offsets: torch.Tensor, k, scale=None) -> torch
     return side
 
 
-def adapt_rounds(g: Grid, xyz, offsets, k, side, inv, target_ratio, rounds=ADAPT_ROUNDS):
-    """Density feedback: each round moves every cloud's cell side by at most
-    4x towards ``target_ratio * k`` candidates per query and rebuilds; stops
-    early once no cloud moves by more than 20 %."""
-    for _ in range(rounds):
-        before = side.clone()
+ADAPT_MIN_POINTS = 16384   # smaller clouds keep the first guess
+
+
+def density_class(mean_candidates, k) -> int:
+    """log2 of (first-guess 27-cell candidates per query) / k: how far the
+    points concentrate below their bounding-box density (surfaces, LiDAR
+    rings); part of the autotuner's bucket key."""
+    r = max(float(mean_candidates) / max(int(k), 1), 1.0)
+    return max(0, min(15, int(math.floor(math.log2(r)))))
+
+
+def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
+                     target_ratio=None, adapt=True, rounds=None, no_sync=True):
+    """Build ``g`` for exact queries of (xyz, offsets); returns the per-cloud
+    cell side.  First guess from the bounding-box density, then (clouds of
+    >= ADAPT_MIN_POINTS points) density-feedback rebuilds towards the
+    autotuned ``target_ratio * k`` candidates per query.  With ``no_sync``
+    the builds use the reserved capacity and read nothing back except the
+    per-cloud first-guess statistic (one read)."""
+    n = xyz.shape[0]
+    if cell is not None:
+        side = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
+        side = side.expand(offsets.shape[0]).contiguous()
+        g.build_xyz(xyz, offsets, (1.0 / side.double()).float(), no_sync=no_sync)
+        return side
+    side, inv = g.auto_cell(xyz, offsets, k, DEFAULT_CELL_SCALE if scale is None else scale)
+    g.build_xyz(xyz, offsets, inv, no_sync=no_sync)
+    if not adapt or n < ADAPT_MIN_POINTS:
+        return side
+    if target_ratio is None:
+        from .autotune import lookup_target_ratio
+        ends = offsets.long()
+        sizes = torch.diff(ends, prepend=ends.new_zeros(1)).clamp_min(1)
+        mean = (g.cloud_candidates().double() / sizes.double()).cpu().tolist()
+        ks = k.tolist() if isinstance(k, torch.Tensor) else [int(k)] * len(mean)
+        ns = sizes.cpu().tolist()
+        target_ratio = torch.tensor(
+            [lookup_target_ratio(nc, kc, density_class(mc, kc))
+             for nc, kc, mc in zip(ns, ks, mean)], dtype=torch.float32, device=xyz.device)
+    for _ in range(ADAPT_ROUNDS if rounds is None else rounds):
         g.adapt_cell(k, side, inv, target_ratio, CANDIDATE_EXPONENT)
-        g.build_xyz(xyz, offsets, inv)
-        change = (side.double() / before.double()).log().abs().max()
-        if float(change) < math.log(1.2):
-            break
+        g.build_xyz(xyz, offsets, inv, no_sync=no_sync)
+    return side
 
 
 class ExactKNN:
@@ -96,30 +131,29 @@ class ExactKNN:
         self.grid = Grid()
         self.last_stats = None
         self.side = None
+        self._cap = (0, 0, 0)
+
+    def _reserve(self, n, B, nq):
+        need = (n, B, nq)
+        if all(a <= c for a, c in zip(need, self._cap)):
+            return
+        n2, B2, q2 = (max(int(a * 1.25) + 16, c) for a, c in zip(need, self._cap))
+        cells = min(128 * n2 + 16 * B2 + 65536, 1 << 28)
+        self.grid.reserve(n2, cells, B2, max(n2, q2))
+        self._cap = (n2, B2, q2)
 
     def build(self, xyz, offsets, k, *, cell=None, scale=None, target_ratio=None,
-              adapt=True):
+              adapt=True, nq=0, no_sync=True):
         """Build the support grid; returns the per-cloud cell side."""
         F.require_device(xyz, "xyz")
         xyz = xyz.float().contiguous()
         offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
-        g = self.grid
-        if cell is not None:
-            side = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
-            inv = (1.0 / side.double()).float()
-            g.build_xyz(xyz, offsets, inv)
-        else:
-            kmax = k if isinstance(k, int) else int(k.max())
-            if target_ratio is None:
-                from .autotune import lookup_target_ratio
-                target_ratio = lookup_target_ratio(xyz.shape[0], kmax)
-            side, inv = g.auto_cell(xyz, offsets, k,
-                                    DEFAULT_CELL_SCALE if scale is None else scale)
-            g.build_xyz(xyz, offsets, inv)
-            if adapt:
-                adapt_rounds(g, xyz, offsets, k, side, inv, target_ratio)
-        self.side = side
-        return side
+        if no_sync:
+            self._reserve(xyz.shape[0], offsets.shape[0], max(nq, xyz.shape[0]))
+        self.side = build_exact_grid(self.grid, xyz, offsets, k, cell=cell, scale=scale,
+                                     target_ratio=target_ratio, adapt=adapt,
+                                     no_sync=no_sync)
+        return self.side
 
     @torch.no_grad()
     def __call__(self, xyz, offsets, k, new_xyz=None, new_offsets=None, *,
@@ -128,23 +162,42 @@ class ExactKNN:
                  target_ratio=None, adapt=True, cut_r2=math.inf, stats=False):
         if k < 1 or k > F.FK_MAX_K:
             raise ValueError(f"k must be in [1, {F.FK_MAX_K}], got {k}")
-        self.build(xyz, offsets, k, cell=cell, scale=scale, target_ratio=target_ratio,
-                   adapt=adapt)
-        g = self.grid
         attached = new_xyz is not None
         if attached:
             if new_offsets is None:
                 raise ValueError("new_offsets required with new_xyz")
             F.require_device(new_xyz, "new_xyz")
-            g.attach_queries(new_xyz.float().contiguous(),
-                             qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32))
         if pad_mode == F.FK_PAD_SELF and attached:
             pad_mode = F.FK_PAD_MINUS1
-        out = g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, cut_r2=cut_r2,
-                      idx_dtype=idx_dtype, pad_mode=pad_mode, dist_mode=dist_mode,
-                      pad_dist=pad_dist, attached=attached)
+        nq = new_xyz.shape[0] if attached else 0
+        try:
+            out = self._run(xyz, offsets, k, new_xyz, new_offsets, attached, nq,
+                            dict(cell=cell, scale=scale, target_ratio=target_ratio,
+                                 adapt=adapt),
+                            dict(cut_r2=cut_r2, idx_dtype=idx_dtype, pad_mode=pad_mode,
+                                 dist_mode=dist_mode, pad_dist=pad_dist), True)
+        except F.FlashKNNError as e:
+            if e.status != F.FK_ERR_GRID_TOO_LARGE:
+                raise
+            # the dense boxes outgrew the reserved capacity: size them exactly
+            out = self._run(xyz, offsets, k, new_xyz, new_offsets, attached, nq,
+                            dict(cell=cell, scale=scale, target_ratio=target_ratio,
+                                 adapt=adapt),
+                            dict(cut_r2=cut_r2, idx_dtype=idx_dtype, pad_mode=pad_mode,
+                                 dist_mode=dist_mode, pad_dist=pad_dist), False)
         if stats:
-            self.last_stats = g.query_stats()
+            self.last_stats = self.grid.query_stats()
+        return out
+
+    def _run(self, xyz, offsets, k, new_xyz, new_offsets, attached, nq, bkw, qkw, no_sync):
+        self.build(xyz, offsets, k, nq=nq, no_sync=no_sync, **bkw)
+        g = self.grid
+        if attached:
+            g.attach_queries(new_xyz.float().contiguous(),
+                             qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32),
+                             no_sync=no_sync)
+        out = g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, attached=attached, **qkw)
+        g.status()  # the one host read: reports any device-side error
         return out
 
 

@@ -167,11 +167,16 @@ int fk_grid_auto_cell(fk_grid *grid, const float *xyz, const int32_t *offsets,
                       void *stream);
 /* Density feedback on a grid built by fk_grid_build_xyz: per cloud, the mean
  * 27-cell candidate count m of the self queries gives
- * side *= clamp((target_ratio * k / m)^(1/exponent), 1/4, 4) and
- * inv = 1/side, in place (device); rebuild with the new inv afterwards. */
+ * side *= clamp((ratio * k / m)^(1/exponent), 1/4, 4) and inv = 1/side, in
+ * place (device), with ratio = target_per_cloud[c] (float[num_clouds],
+ * device, nullable) or target_ratio; rebuild with the new inv afterwards. */
 int fk_grid_adapt_cell(fk_grid *grid, const int32_t *k_per_cloud, int32_t k,
-                       float target_ratio, float exponent, float *side_io,
-                       float *inv_io, void *stream);
+                       float target_ratio, const float *target_per_cloud,
+                       float exponent, float *side_io, float *inv_io,
+                       void *stream);
+/* Per-cloud sum of the 27-cell candidate counts of the self queries
+ * (int64[num_clouds], device; no host synchronisation). */
+int fk_grid_cloud_candidates(fk_grid *grid, int64_t *out, void *stream);
 
 /* Attach a query list in the grid's cell space.  For grids built from
  * cells: qcell (int64[nq,3], device) are voxel coordinates (shifted by the
@@ -186,7 +191,8 @@ int fk_grid_attach_queries(fk_grid *grid, const float *qxyz,
                            const int32_t *qpad, int32_t flags, void *stream);
 
 /* Device status word of the last build (FK_OK or the first error a kernel
- * raised); reading it synchronises the stream. */
+ * raised); reading it synchronises the stream and makes the cell count of
+ * an FK_BUILD_NO_SYNC build known to fk_grid_info. */
 int fk_grid_status(fk_grid *grid, void *stream);
 /* Host-side sizes of the last build (points, clouds, cells; -1 if unknown
  * in FK_BUILD_NO_SYNC mode). */

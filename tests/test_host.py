@@ -82,3 +82,19 @@ def test_pack_offsets():
     from flashknn_exprepo_amd import synthetic
     xyz, off = synthetic.pack([np.zeros((3, 3)), np.ones((5, 3))])
     assert off.tolist() == [3, 8] and xyz.shape == (8, 3)
+
+
+def test_autotune_lookup_and_density_class(tmp_path, monkeypatch):
+    import json
+    from flashknn_exprepo_amd import autotune
+    from flashknn_exprepo_amd.knn import DEFAULT_TARGET_RATIO, density_class
+    assert density_class(16 * 9, 16) == 3 and density_class(3, 16) == 0
+    table = {autotune.bucket(100000, 16, 3): {"target_ratio": 13.0},
+             autotune.bucket(10000, 16, 3): {"target_ratio": 6.5}}
+    p = tmp_path / "t.json"
+    p.write_text(json.dumps(table))
+    monkeypatch.setattr(autotune, "TABLE_PATH", str(p))
+    monkeypatch.setattr(autotune, "_TABLE", None)
+    assert autotune.lookup_target_ratio(100000, 16, 3) == 13.0
+    assert autotune.lookup_target_ratio(12000, 16, 3) == 6.5      # nearest size
+    assert autotune.lookup_target_ratio(100000, 32, 3) == DEFAULT_TARGET_RATIO
