@@ -12,7 +12,9 @@ and streams.  Public surface:
 * ``BatchedKNN`` - many uneven clouds, per-cloud k, one persistent launch,
   hipGraph replay;
 * ``gathered_sqdist`` / ``idw_interpolate`` - differentiable distances and
-  inverse-distance-weighted interpolation with hand-written backward.
+  inverse-distance-weighted interpolation with hand-written backward;
+* ``voxel_first`` / ``build_flash_hierarchy`` - voxel-first subsampling on
+  the device and the level hierarchy built from it.
 """
 from . import pointops
 from ._lib import FlashKNNError, load
@@ -21,11 +23,13 @@ from .batched import BatchedKNN
 from .functions import (AdaptiveNeighborhoodFlashKNN, FlashKNN, build_adaptive_octree,
                         cal_recall, select_adaptive_levels, vanilla_Knn, xyz2key)
 from .grid import Grid
+from .hierarchy import build_flash_hierarchy, voxel_first
 from .knn import ExactKNN, cell_sizes, knn
 
 __all__ = [
     "FlashKNN", "AdaptiveNeighborhoodFlashKNN", "build_adaptive_octree",
     "select_adaptive_levels", "xyz2key", "cal_recall", "vanilla_Knn", "pointops", "knn",
     "ExactKNN", "cell_sizes", "BatchedKNN", "Grid", "gathered_sqdist",
-    "idw_gather", "idw_interpolate", "FlashKNNError", "load",
+    "idw_gather", "idw_interpolate", "FlashKNNError", "load", "voxel_first",
+    "build_flash_hierarchy",
 ]

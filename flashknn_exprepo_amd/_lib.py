@@ -24,7 +24,8 @@ FK_ERR_DOMAIN = 4
 FK_ERR_HIP = 5
 FK_ERR_CAPACITY = 6
 FK_ERR_STATE = 7
-FK_MAX_K = 64
+FK_MAX_K = 128
+FK_MAX_BATCHED_K = 64
 
 FK_MODE_CELLS = 0
 FK_MODE_EXACT = 1
@@ -111,6 +112,8 @@ SIGNATURES = [
     ("fk_batched_query", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp, _vp, _vp]),
     ("fk_batched_capture", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp, _vp, _vp]),
     ("fk_graph_replay", ctypes.c_int, [_vp, _vp]),
+    ("fk_voxel_workspace_bytes", ctypes.c_int64, [_i64]),
+    ("fk_voxel_first", ctypes.c_int, [_vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp, _vp, _vp]),
     ("fk_backward_workspace_bytes", ctypes.c_int64, [_i64, _i64, _i32]),
     ("fk_dist_backward", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("fk_idw_forward", ctypes.c_int, [_vp, _i64, _i32, _vp, _vp, _i64, _i32, _vp, _vp]),

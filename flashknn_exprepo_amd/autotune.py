@@ -28,7 +28,7 @@ RATIOS = (4.5, 6.5, 9.0, 13.0, 19.0, 28.0)
 
 
 def _kclass(k: int) -> int:
-    return 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 64
+    return 8 if k <= 8 else 16 if k <= 16 else 32 if k <= 32 else 64 if k <= 64 else 128
 
 
 def bucket(n: int, k: int, dclass: int = 0) -> str:

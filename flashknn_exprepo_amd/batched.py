@@ -31,8 +31,8 @@ class BatchedKNN:
         if ks.shape[0] != self.offsets.shape[0]:
             raise ValueError("one k per cloud")
         kmin, kmax = int(ks.min()), int(ks.max())
-        if kmin < 1 or kmax > F.FK_MAX_K:
-            raise ValueError(f"k must be in [1, {F.FK_MAX_K}]")
+        if kmin < 1 or kmax > F.FK_MAX_BATCHED_K:
+            raise ValueError(f"k must be in [1, {F.FK_MAX_BATCHED_K}]")
         self.k = ks
         self.max_k = kmax
         self.exact = exact

@@ -220,6 +220,12 @@ int status_message(int st) {
     case FK_ERR_GRID_TOO_LARGE:
       return fail(st, "the dense per-cloud cell boxes exceed the cell capacity; "
                       "use a larger cell size (or reserve more cells)");
+    case FK_ERR_K_RANGE:
+      return fail(st, "a per-cloud k is outside [1, params->k] of the batched "
+                      "launch");
+    case FK_ERR_STATE:
+      return fail(st, "the pending-query counter was not reset before a "
+                      "first pass (passes mask without bit 0)");
     default:
       return fail(st, "device status %d", st);
   }
@@ -340,7 +346,7 @@ int fk_grid_reserve(fk_grid *g, int64_t max_points, int64_t max_cells,
   FK_CHECK(ensure_qcells(g, max_cells));
   const int64_t nq = max_queries > max_points ? max_queries : max_points;
   FK_CHECK(ensure_queries(g, nq));
-  FK_CHECK(ensure_pending(g, nq, FK_MAX_K));
+  FK_CHECK(ensure_pending(g, nq, kMaxLaneK));
   g->reserved = 1;
   return ok();
 }
@@ -539,7 +545,7 @@ int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
   FK_CHECK(check_stream_device(g));
   QueryArgs a;
   FK_CHECK(make_args(g, p, out_idx, out_dist, a));
-  if (a.exact) {
+  if (a.exact && a.k <= kMaxLaneK) {
     // grows only when the reserve (or the last query) was smaller
     FK_CHECK(ensure_pending(g, a.nq, a.k));
     a.pend_q = g->pend_q;
@@ -678,6 +684,9 @@ static int batched_args(fk_grid *g, const fk_query_params *p,
   if (!k_per_cloud || !out_offsets)
     return fail(FK_ERR_INVALID_ARG, "fk_batched_query: k_per_cloud and "
                                     "out_offsets required");
+  if (p->k > FK_MAX_BATCHED_K)
+    return fail(FK_ERR_K_RANGE, "fk_batched_query: k = %d above %d", p->k,
+                FK_MAX_BATCHED_K);
   if (a.exact) {
     FK_CHECK(ensure_pending(g, a.nq, a.k));
     a.pend_q = g->pend_q;
@@ -739,6 +748,26 @@ int fk_batched_capture(fk_grid *g, const fk_query_params *p,
 int fk_graph_replay(fk_grid *g, void *stream) {
   if (!g || !g->gexec) return fail(FK_ERR_STATE, "fk_graph_replay: nothing captured");
   FK_HIP(hipGraphLaunch(g->gexec, S(stream)));
+  return ok();
+}
+
+int64_t fk_voxel_workspace_bytes(int64_t n) {
+  if (n < 0 || n > INT32_MAX / 2) return -1;
+  return voxel_workspace_bytes(n);
+}
+
+int fk_voxel_first(const float *xyz, const int32_t *offsets, int32_t num_clouds,
+                   int64_t n, float voxel_size, void *workspace,
+                   int64_t *out_idx, int64_t *out_count, void *stream) {
+  if (n < 0 || n > INT32_MAX / 2 || num_clouds < 1 ||
+      (!offsets && num_clouds != 1) || (n > 0 && (!xyz || !out_idx)) ||
+      !workspace || !out_count)
+    return fail(FK_ERR_INVALID_ARG, "fk_voxel_first: bad arguments");
+  if (!(voxel_size > 0.f) || !std::isfinite(voxel_size))
+    return fail(FK_ERR_INVALID_ARG,
+                "fk_voxel_first: voxel_size must be finite and > 0");
+  FK_HIP(launch_voxel_first(xyz, offsets, num_clouds, n, voxel_size, workspace,
+                            out_idx, out_count, S(stream)));
   return ok();
 }
 

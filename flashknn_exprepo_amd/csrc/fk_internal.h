@@ -9,6 +9,9 @@
 namespace fk {
 
 constexpr int kWave = 64;
+// k above this runs one wave per query (fk_query) and is refused by the
+// batched kernel
+constexpr int kMaxLaneK = 64;
 
 // Slots of the small device scalar block owned by every grid handle.
 enum Scalar : int {
@@ -145,7 +148,7 @@ struct QueryArgs {
 hipError_t launch_query(const QueryArgs &a, hipStream_t s);
 hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s);
 
-int kclass_of(int k);  // 8, 16, 32 or 64
+int kclass_of(int k);  // 8, 16, 32, 64 or 128
 hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
                                   hipStream_t s);
 hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,
@@ -168,6 +171,13 @@ hipError_t launch_scan_host(const int32_t *in, int32_t *out, int64_t n,
                             int32_t *part, hipStream_t s);
 
 // backward
+// voxel-first subsampling
+int64_t voxel_workspace_bytes(int64_t n);
+hipError_t launch_voxel_first(const float *xyz, const int32_t *offsets,
+                              int32_t B, int64_t n, float size, void *ws,
+                              int64_t *out_idx, int64_t *out_count,
+                              hipStream_t s);
+
 int64_t backward_workspace_bytes(int64_t nq, int64_t ns, int32_t k);
 hipError_t launch_dist_backward(const float *q, int64_t nq, const float *p,
                                 int64_t ns, const int64_t *idx, int32_t k,

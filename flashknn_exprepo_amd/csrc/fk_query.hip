@@ -509,22 +509,54 @@ __device__ __forceinline__ uint64_t wave_merge(uint64_t L, uint64_t S, int lane)
   return v;
 }
 
+// Bitonic sequence across the wave -> ascending (butterfly of 6 stages).
+__device__ __forceinline__ uint64_t wave_bitonic_merge(uint64_t v, int lane) {
+#pragma unroll
+  for (int j = kWave >> 1; j > 0; j >>= 1) {
+    const uint64_t o = __shfl_xor(v, j, kWave);
+    v = (lane & j) == 0 ? umin64(v, o) : umax64(v, o);
+  }
+  return v;
+}
+
+// Best list of P * 64 keys: L[p] at lane i holds rank p * 64 + i.
+template <int P>
 struct CoopList {
-  uint64_t L;       // this lane's rank in the sorted best list
+  uint64_t L[P];
   uint64_t *buf;    // the wave's LDS buffer
   int cnt;          // buffered keys (wave-uniform)
   int k;
   uint64_t cutkey, thr;
 
+  __device__ __forceinline__ uint64_t kth() const {
+    if constexpr (P == 1) {
+      return __shfl(L[0], k - 1, kWave);
+    } else {
+      // k is wave-uniform: one of the two reads runs
+      return k <= kWave ? __shfl(L[0], k - 1, kWave) : __shfl(L[1], k - 1 - kWave, kWave);
+    }
+  }
   __device__ __forceinline__ void refresh(int lane) {
     (void)lane;
-    const uint64_t kth = __shfl(L, k - 1, kWave);
-    thr = kth < cutkey ? kth : cutkey;
+    const uint64_t kk = kth();
+    thr = kk < cutkey ? kk : cutkey;
   }
   __device__ __forceinline__ void flush(int lane) {
     for (int c0 = 0; c0 < cnt; c0 += kWave) {
       const uint64_t v = c0 + lane < cnt ? buf[c0 + lane] : kMaxKey;
-      L = wave_merge(L, wave_sort(v, lane), lane);
+      const uint64_t S = wave_sort(v, lane);
+      if constexpr (P == 1) {
+        L[0] = wave_merge(L[0], S, lane);
+      } else {
+        // 128 smallest of L (128, ascending) and S (64, ascending; padded
+        // with +inf to 128): C[i] = min(L[i], S'[127 - i]) is bitonic with
+        // C[0..63] = L[0..63]; the first merge stage (distance 64) is the
+        // in-lane min/max, then each half merges across the wave.
+        const uint64_t c1 = umin64(L[1], __shfl(S, kWave - 1 - lane, kWave));
+        const uint64_t lo = umin64(L[0], c1), hi = umax64(L[0], c1);
+        L[0] = wave_bitonic_merge(lo, lane);
+        L[1] = wave_bitonic_merge(hi, lane);
+      }
     }
     cnt = 0;
     refresh(lane);
@@ -541,8 +573,10 @@ struct CoopList {
   }
 };
 
-// Columns of the shell Ro < Chebyshev distance <= Rn, split over the lanes.
-__device__ void coop_shell(CoopList &t, const QueryArgs &a, const CloudInfo &ci,
+// Columns of the shell Ro < Chebyshev distance <= Rn, split over the lanes
+// (Ro = -1: the whole block of half-width Rn).
+template <int P>
+__device__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &ci,
                            int4 qc, float4 qr, int Ro, int Rn, int lane) {
   const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
   const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
@@ -593,6 +627,50 @@ __device__ void coop_shell(CoopList &t, const QueryArgs &a, const CloudInfo &ci,
   if (t.cnt > 0) t.flush(lane);
 }
 
+// One output entry (index + distance) from a key.
+__device__ __forceinline__ void write_entry(const QueryArgs &a, int64_t o,
+                                            uint64_t key) {
+  const int id = (int)(uint32_t)key;
+  if (a.idx64)
+    ((int64_t *)a.out_idx)[o] = (int64_t)id;
+  else
+    ((int32_t *)a.out_idx)[o] = id;
+  if (a.out_d) {
+    const uint32_t db = (uint32_t)(key >> 32);
+    float d = __uint_as_float(db);
+    if (db == kInfBits)
+      d = a.pad_dist;
+    else if (a.dist_mode == FK_DIST_SQRT)
+      d = sqrtf(d);
+    a.out_d[o] = d;
+  }
+}
+
+// min(k-th distance, cut) of a cooperative list.
+template <int P>
+__device__ __forceinline__ float coop_eff(const CoopList<P> &t, float cut_r2) {
+  const float e = __uint_as_float((uint32_t)(t.kth() >> 32));
+  return e < cut_r2 ? e : cut_r2;
+}
+
+// Grow the block from half-width Ro shell by shell until the query
+// certifies or the block covers the cloud's box.  Returns the final Ro.
+template <int P>
+__device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
+                                           const CloudInfo &ci, int4 qc, float4 qr,
+                                           int Ro, int lane) {
+  const int Rcover = cover_radius(ci, qc);
+  for (;;) {  // wave-uniform
+    double umax;
+    const double gap = cert_gap(ci, qc, qr, Ro, umax);
+    if (certified(gap, coop_eff(t, a.cut_r2), ci.inv_cell, umax, Ro) || Ro >= Rcover)
+      return Ro;
+    const int Rn = next_radius(ci, qc, qr, Ro, coop_eff(t, a.cut_r2), Rcover);
+    coop_shell<P>(t, a, ci, qc, qr, Ro, Rn, lane);
+    Ro = Rn;
+  }
+}
+
 __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
@@ -605,50 +683,60 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
     const int4 qc = a.qcell[qs];
     const CloudInfo ci = a.clouds[qc.w];
     const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
-    CoopList t;
+    CoopList<1> t;
     t.buf = sbuf + w * kCoopCap;
     t.cnt = 0;
     t.k = k;
     t.cutkey = make_key(a.cut_r2, 0u);
-    t.L = lane < k ? a.pend_keys[(int64_t)slot * a.k + lane] : kMaxKey;
+    t.L[0] = lane < k ? a.pend_keys[(int64_t)slot * a.k + lane] : kMaxKey;
     t.refresh(lane);
+    // the first pass left it uncertified at half-width a.R
     const int Rcover = cover_radius(ci, qc);
-    int Ro = a.R;
-    while (Ro < Rcover) {  // wave-uniform
-      const uint64_t kk = __shfl(t.L, k - 1, kWave);
-      float eff = __uint_as_float((uint32_t)(kk >> 32));
-      eff = eff < a.cut_r2 ? eff : a.cut_r2;
-      const int Rn = next_radius(ci, qc, qr, Ro, eff, Rcover);
-      coop_shell(t, a, ci, qc, qr, Ro, Rn, lane);
-      Ro = Rn;
-      double umax;
-      const double gap = cert_gap(ci, qc, qr, Ro, umax);
-      const uint64_t k2 = __shfl(t.L, k - 1, kWave);
-      float eff2 = __uint_as_float((uint32_t)(k2 >> 32));
-      eff2 = eff2 < a.cut_r2 ? eff2 : a.cut_r2;
-      if (certified(gap, eff2, ci.inv_cell, umax, Ro)) break;
+    if (a.R < Rcover) {
+      const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(t, a.cut_r2), Rcover);
+      coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane);
+      coop_expand<1>(t, a, ci, qc, qr, Rn, lane);
     }
     ++nexp;
-    if (lane < k) {
-      const int row = __float_as_int(qr.w);
-      const int64_t o = out_base(a, ci, qc.w, row, k) + lane;
-      const uint64_t key = t.L;
-      const int id = (int)(uint32_t)key;
-      if (a.idx64)
-        ((int64_t *)a.out_idx)[o] = (int64_t)id;
-      else
-        ((int32_t *)a.out_idx)[o] = id;
-      if (a.out_d) {
-        const uint32_t db = (uint32_t)(key >> 32);
-        float d = __uint_as_float(db);
-        if (db == kInfBits)
-          d = a.pad_dist;
-        else if (a.dist_mode == FK_DIST_SQRT)
-          d = sqrtf(d);
-        a.out_d[o] = d;
-      }
-    }
+    if (lane < k)
+      write_entry(a, out_base(a, ci, qc.w, __float_as_int(qr.w), k) + lane, t.L[0]);
   }
+  if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
+}
+
+// k in (64, 128]: one wave per query from the start, the best list held as
+// two keys per lane.  Scans the block of half-width a.R; FK_MODE_EXACT then
+// grows it until the query certifies.
+__global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
+  __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  const int nq = a.scalars[kStatus] != 0 ? 0 : a.nq;
+  const int nwaves = gridDim.x * kCoopWaves;
+  int ncert = 0, nexp = 0;
+  for (int qs = blockIdx.x * kCoopWaves + w; qs < nq; qs += nwaves) {
+    const float4 qr = a.qrec[qs];
+    const int4 qc = a.qcell[qs];
+    const CloudInfo ci = a.clouds[qc.w];
+    const int k = a.k;
+    CoopList<2> t;
+    t.buf = sbuf + w * kCoopCap;
+    t.cnt = 0;
+    t.k = k;
+    t.cutkey = make_key(a.cut_r2, 0u);
+    const uint64_t padkey = ((uint64_t)kInfBits << 32) | pad_value(a, qr, qs);
+    t.L[0] = lane < k ? padkey : kMaxKey;
+    t.L[1] = lane + kWave < k ? padkey : kMaxKey;
+    t.refresh(lane);
+    coop_shell<2>(t, a, ci, qc, qr, -1, a.R, lane);
+    if (a.exact) {
+      const int Ro = coop_expand<2>(t, a, ci, qc, qr, a.R, lane);
+      if (Ro > a.R) ++nexp; else ++ncert;
+    }
+    const int64_t o = out_base(a, ci, qc.w, __float_as_int(qr.w), k);
+    if (lane < k) write_entry(a, o + lane, t.L[0]);
+    if (lane + kWave < k) write_entry(a, o + kWave + lane, t.L[1]);
+  }
+  if (lane == 0 && ncert) atomicAdd(a.scalars + kStatCertified, ncert);
   if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
 }
 
@@ -686,6 +774,11 @@ k_batched(QueryArgs a) {
     const int qe = a.cell_start[ci.base + ci.nx * ci.ny * ci.nz];
     const int q0 = qb + (item - prefix[c]) * kWave;
     const int k = a.k_per_cloud[c];
+    if (k < 1 || k > a.k) {
+      // a per-cloud k above the launch's k class: refuse the tile
+      if (lane == 0) raise_status(a.scalars, FK_ERR_K_RANGE);
+      continue;
+    }
     if (k <= 8 || MAXKC == 8)
       run_tile<8>(a, q0, qe, lane_buf);
     else if (k <= 16 || MAXKC == 16)
@@ -884,12 +977,22 @@ hipError_t launch_batched_t(const QueryArgs &a, hipStream_t s) {
 
 }  // namespace
 
-int kclass_of(int k) { return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 : 64)); }
+int kclass_of(int k) {
+  return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 : (k <= 64 ? 64 : 128)));
+}
 
 hipError_t launch_query(const QueryArgs &a, hipStream_t s) {
   if (a.passes & 1)
     hipLaunchKernelGGL(k_reset_query_scalars, dim3(1), dim3(64), 0, s, a.scalars);
   if (a.nq <= 0) return hipGetLastError();
+  if (a.k > kMaxLaneK) {
+    if (a.passes & 2) {
+      const int64_t b = ((int64_t)a.nq + kCoopWaves - 1) / kCoopWaves;
+      const unsigned nb = (unsigned)(b < 16384 ? b : 16384);
+      hipLaunchKernelGGL(k_query_wide, dim3(nb), dim3(64 * kCoopWaves), 0, s, a);
+    }
+    return hipGetLastError();
+  }
   switch (kclass_of(a.k)) {
     case 8: return launch_plain<8>(a, s);
     case 16: return launch_plain<16>(a, s);

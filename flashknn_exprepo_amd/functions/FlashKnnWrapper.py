@@ -20,7 +20,8 @@ its scan order met); memory_mode / sorting_mode / candidate_mode /
 enable_skip / thread_group_size / dynamic_load / batch_for_prune are
 validated exactly as the reference validates them and then select the same
 kernel (one wave64 design replaces the SM/GM/PS/SS ablations), so results
-are identical across modes and always sorted; K ranges over 1..64;
+are identical across modes and always sorted; K ranges over 1..128 (above
+64 one wave per query);
 back_query with K > 1 returns sorted rows padded with (-1, inf).
 """
 from __future__ import annotations
@@ -140,7 +141,7 @@ class FlashKNN:
                    query_grid_size: float, down_grid_size: float,
                    batch_idx: torch.Tensor, dynamic_load=True, memory_mode="SM"):
         F.require_device(xyz, "xyz")
-        grid_coord = (xyz / query_grid_size).long()
+        grid_coord = quotient32(xyz, query_grid_size).long()  # FlashKnnWrapper.py:480
         sel = sel_indices.long()
         g = self._grid_handle()
         sb = None if batch_idx is None else batch_idx[sel]
@@ -158,6 +159,16 @@ class FlashKNN:
         if self.num_nbr == 1:
             return idx.view(-1)
         return idx, dist
+
+
+def quotient32(x: torch.Tensor, size: float) -> torch.Tensor:
+    """fl32(x / fl32(size)): the correctly rounded fp32 quotient of
+    ``x / size`` for an fp32 tensor and a Python scalar.  Evaluated in
+    float64 and rounded once to fp32, which gives the correctly rounded
+    result for a quotient of fp32 values (53 >= 2 * 24 + 2 bits), so the
+    cells do not depend on how a torch build implements fp32 division."""
+    s = float(torch.tensor(size, dtype=torch.float32))
+    return (x.double() / s).float()
 
 
 def _validate_modes(memory_mode, sorting_mode, candidate_mode, enable_skip,

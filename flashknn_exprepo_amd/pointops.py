@@ -7,8 +7,9 @@ interpolation.py, utils.py) on the exact grid kNN.
   cumulative segment ends, idx are global indices into xyz, dist is the
   square root of the squared distance, missing slots are (-1, sqrt(1e10)).
   Results are exact kNN with ties to the smaller index (the reference's heap
-  order among equal distances is not specified).  nsample <= 64 (the
-  reference allows 128).
+  order among equal distances is not specified).  nsample <= 128, the
+  reference's limit (knn_query_cuda_kernel.cu:82-83); above 64 each query
+  runs on a whole wave.
 * ``interpolation`` / ``interpolation2`` (interpolation.py:8-59): inverse-
   distance weights 1/(dist + 1e-8) over the k nearest, normalised; the
   backward returns the feature gradient through a deterministic segmented

@@ -68,7 +68,12 @@ typedef enum fk_status {
   FK_ERR_STATE = 7           /* handle not built / no queries attached    */
 } fk_status;
 
-#define FK_MAX_K 64
+/* fk_query: k <= 64 keeps each query's list in one lane's registers;
+ * 65..128 (pointops nsample <= 128,
+ * Pointcept/libs/pointops/src/knn_query/knn_query_cuda_kernel.cu) runs one
+ * wave per query.  The batched kernel takes per-cloud k <= 64. */
+#define FK_MAX_K 128
+#define FK_MAX_BATCHED_K 64
 
 /* fk_query_params.mode */
 #define FK_MODE_CELLS 0 /* candidates = support points of the query's cloud
@@ -266,6 +271,24 @@ int fk_idw_backward(const float *feat, int64_t ns, int32_t channels,
                     const int64_t *idx, const float *w, int64_t nq, int32_t k,
                     const float *grad_out, float *grad_feat, float *grad_w,
                     void *workspace, void *stream);
+
+/* -------------------------------------------- voxel-first subsampling */
+/* Replaces _voxel_first_cuda (Networks/hierarchy.py:23-36), the step that
+ * builds every coarser level of the hierarchy before the kNN path.  Voxel
+ * v = floor(x / voxel_size) per axis (correctly rounded fp32 quotient, as
+ * torch divides a device tensor by a Python scalar), keyed by its low 16
+ * bits per axis (FlashKNN/functions/z_order.py:56-91) and by its cloud;
+ * the voxel's representative is its smallest point index.  Writes the kept
+ * indices ascending to out_idx (int64[n], device) and their count to
+ * *out_count (int64, device; -status on a device-side error: a coordinate
+ * outside |v| < 2^31 or not finite, a point past offsets[num_clouds-1]).
+ * offsets: int32[num_clouds] cumulative ends (device) or NULL for one cloud.
+ * No host synchronisation; workspace (device) must hold
+ * fk_voxel_workspace_bytes(n) bytes. */
+int64_t fk_voxel_workspace_bytes(int64_t n);
+int fk_voxel_first(const float *xyz, const int32_t *offsets, int32_t num_clouds,
+                   int64_t n, float voxel_size, void *workspace,
+                   int64_t *out_idx, int64_t *out_count, void *stream);
 
 #ifdef __cplusplus
 }

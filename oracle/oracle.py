@@ -25,6 +25,7 @@ Reference citations:
   padding (self,inf)  dynamic_load.cu:562; FlashKnnWrapper.py:427-429
   selected query      FlashKnnWrapper.py:646-849
   back query          FlashKnnWrapper.py:460-644 (cells = trunc(xyz/size))
+  voxel-first         Networks/hierarchy.py:23-36 (first index per voxel key)
   pointops knn        Pointcept/libs/pointops/src/knn_query/
                       knn_query_cuda_kernel.cu:60-104; functions/query.py:7-24
   IDW                 Pointcept/libs/pointops/functions/interpolation.py:8-56
@@ -166,15 +167,53 @@ def selected_query(xyz, grid_coord, query_indices, batch_idx, k, num_down,
 
 def back_query(xyz, sel_indices, query_grid_size, batch_idx, k):
     """FlashKNN.back_query (FlashKnnWrapper.py:460-644): every point of xyz
-    queries the subset xyz[sel_indices]; cells are trunc(xyz/query_grid_size)
-    (``.long()`` truncates, :480); returned indices are positions in
-    sel_indices.  Padding (-1, +inf) (dynamic_load.cu:1098-1102)."""
+    queries the subset xyz[sel_indices]; cells are (xyz/query_grid_size).long()
+    (:480): trunc of the correctly rounded fp32 quotient; returned indices
+    are positions in sel_indices.  Padding (-1, +inf)
+    (dynamic_load.cu:1098-1102)."""
     pts = _f32(xyz).reshape(-1, 3)
     cell = np.trunc(pts / np.float32(query_grid_size)).astype(np.int64)
     sel = _i64(sel_indices).reshape(-1)
     sb = None if batch_idx is None else _i64(batch_idx)[sel]
     pad = np.full(len(pts), -1, np.int64)
     return grid_knn(pts[sel], cell[sel], sb, pts, cell, batch_idx, pad, k, 1)
+
+
+# ------------------------------------------------- voxel-first subsampling
+
+def voxel_keys(xyz, size, offsets=None):
+    """Voxel key of every point: (cloud, low 16 bits of floor(x / size) per
+    axis, fp32 quotient correctly rounded as torch.floor(xyz / size) divides).  The reference's key is xyz2key of the voxel coordinates
+    (FlashKNN/functions/z_order.py:56-91, depth 16: bits 0..15 of each axis,
+    a bijection of them), so equal keys here <=> equal keys there."""
+    pts = _f32(xyz).reshape(-1, 3)
+    v = np.floor(pts / np.float32(size)).astype(np.int64) & 0xFFFF
+    cloud = np.zeros(len(pts), np.int64)
+    if offsets is not None:
+        cloud = np.searchsorted(_i64(offsets).reshape(-1), np.arange(len(pts)), side="right")
+    return (cloud << 48) | (v[:, 0] << 32) | (v[:, 1] << 16) | v[:, 2]
+
+
+def voxel_first(xyz, size, offsets=None):
+    """_voxel_first_cuda (Networks/hierarchy.py:23-36): a stable argsort of
+    the voxel keys, the first of every run (the smallest index of each
+    voxel), returned ascending.  np.unique's return_index is that first
+    occurrence."""
+    key = voxel_keys(xyz, size, offsets)
+    _, first = np.unique(key, return_index=True)
+    return np.sort(first).astype(np.int64)
+
+
+def voxel_first_python(xyz, size, offsets=None):
+    """Pure-Python loop form of voxel_first (small inputs): scan the points in
+    index order and keep each voxel's first."""
+    key = voxel_keys(xyz, size, offsets)
+    seen, keep = set(), []
+    for i, k in enumerate(key.tolist()):
+        if k not in seen:
+            seen.add(k)
+            keep.append(i)
+    return np.array(keep, np.int64)
 
 
 # ---------------------------------------------------- exact brute force kNN

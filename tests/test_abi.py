@@ -68,3 +68,18 @@ def test_no_cpu_fallback():
     import flashknn_exprepo_amd as fk
     with pytest.raises(ValueError, match="HIP device"):
         fk.knn(torch.zeros(4, 3), torch.tensor([4], dtype=torch.int32), 2)
+
+
+def test_header_constants_match_python():
+    """Every integer #define and fk_status value of the header has the same
+    value in the Python binding."""
+    from flashknn_exprepo_amd import _lib
+    src = open(os.path.join(ROOT, "include", "flashknn.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    consts = dict(re.findall(r"#define\s+(FK_[A-Z0-9_]+)\s+(\d+)\b", src))
+    consts.update(re.findall(r"\b(FK_(?:OK|ERR_[A-Z_]+))\s*=\s*(\d+)", src))
+    assert "FK_MAX_K" in consts and "FK_ERR_STATE" in consts
+    for name, v in consts.items():
+        if name == "FK_ABI_VERSION":
+            continue
+        assert getattr(_lib, name) == int(v), name

@@ -36,7 +36,7 @@ def _same(gpu_idx, gpu_d, ref_idx, ref_d):
 
 # ------------------------------------------------ FlashKNN.query (27 cells)
 
-@pytest.mark.parametrize("k", [1, 2, 8, 16, 24, 32, 48, 64])
+@pytest.mark.parametrize("k", [1, 2, 8, 16, 24, 32, 48, 64, 96, 128])
 def test_query_uniform_cells(oracle, k):
     xyz = np.random.default_rng(47).random((8192, 3), dtype=np.float32)
     grid = np.floor(xyz / np.float32(0.05)).astype(np.int64)
@@ -110,7 +110,7 @@ def test_query_mode_validation():
     with pytest.raises(ValueError):
         knn.query(g, None, None, thread_group_size=12)
     with pytest.raises(ValueError):
-        fk.FlashKNN(num_nbr=65, num_down=0).query(g, None, None)
+        fk.FlashKNN(num_nbr=129, num_down=0).query(g, None, None)
     with pytest.raises(ValueError):
         knn.query(g, _t(np.array([0, 1, -1, 0])), None)
 
@@ -171,7 +171,7 @@ def test_back_query(oracle, k):
 
 # ------------------------------------------------------- exact kNN
 
-@pytest.mark.parametrize("k", [1, 8, 16, 32, 64])
+@pytest.mark.parametrize("k", [1, 8, 16, 32, 64, 65, 100])
 def test_exact_uniform_vs_brute(oracle, k):
     xyz = np.random.default_rng(47).random((8192, 3), dtype=np.float32)
     ref_i, ref_d = oracle.brute_knn(xyz, [8192], xyz, [8192], k)
@@ -216,6 +216,65 @@ def test_pointops_knn_query(oracle):
     assert idx.dtype == torch.int32 and dist.dtype == torch.float32
     assert np.array_equal(idx.cpu().numpy(), ref_i)
     np.testing.assert_array_equal(dist.cpu().numpy(), np.sqrt(ref_d))
+
+
+@pytest.mark.parametrize("k", [65, 100, 128])
+def test_pointops_knn_query_wide(oracle, k):
+    """nsample above 64 (one wave per query) up to the reference's 128; the
+    90-point segment pads with (-1, sqrt(1e10)); attached queries too."""
+    rng = np.random.default_rng(17)
+    s = rng.random((5000, 3), dtype=np.float32)
+    off = np.array([90, 2000, 5000], np.int32)
+    q = rng.random((700, 3), dtype=np.float32)
+    qoff = np.array([100, 400, 700], np.int32)
+    for new_xyz, new_off in ((None, None), (q, qoff)):
+        qq, qo = (s, off) if new_xyz is None else (new_xyz, new_off)
+        ref_i, ref_d = oracle.brute_knn(s, off, qq, qo, k)
+        if new_xyz is None:
+            idx, dist = fk.pointops.knn_query(k, _t(s), _t(off))
+        else:
+            idx, dist = fk.pointops.knn_query(k, _t(s), _t(off), _t(q), _t(qoff))
+        i_np, d_np = idx.cpu().numpy(), dist.cpu().numpy()
+        assert np.array_equal(i_np, ref_i)
+        pad = ref_i < 0
+        assert not pad[:qo[0], :90].any() and pad[:qo[0], 90:].all()
+        assert not pad[qo[0]:].any()
+        np.testing.assert_array_equal(d_np[~pad], np.sqrt(ref_d[~pad]))
+        assert (d_np[pad] == np.float32(1e5)).all()
+
+
+def test_exact_wide_k_expands(oracle):
+    """k = 96 with cells far too small for it: every query expands."""
+    from flashknn_exprepo_amd import synthetic
+    rng = np.random.default_rng(18)
+    a = synthetic.subsample_to(synthetic.room(seed=48, target=30000), 6000, rng)
+    b = synthetic.subsample_to(synthetic.lidar(seed=49), 7000, rng)
+    xyz, off = synthetic.pack([a, b])
+    ref_i, ref_d = oracle.brute_knn(xyz, off, xyz, off, 96)
+    eng = fk.ExactKNN()
+    idx, d = eng(_t(xyz), _t(off), 96, cell=torch.tensor([0.02, 0.05], device=_dev()),
+                 stats=True)
+    _same(idx, d, ref_i, ref_d)
+    st = eng.last_stats
+    assert st["certified_first_pass"] + st["expanded"] == len(xyz) and st["expanded"] > 0
+
+
+def test_batched_k_range_errors():
+    """Batched k is capped at 64 on the host; a per-cloud k above the
+    launch's k is refused on the device (the kernel skips those tiles)."""
+    from flashknn_exprepo_amd import synthetic
+    from flashknn_exprepo_amd import _lib as F
+    clouds, ks = _batch_case(19)
+    xyz, off = synthetic.pack(clouds)
+    with pytest.raises(ValueError):
+        fk.BatchedKNN(_t(xyz), _t(off), [8, 100, 8, 8, 8, 8])
+    bk = fk.BatchedKNN(_t(xyz), _t(off), ks, exact=True)
+    with pytest.raises(ValueError):
+        bk.grid.batched_query(bk.k, bk.out_offsets, bk.out_idx, bk.out_dist, max_k=16)
+        bk.grid.status()
+    # the error is sticky on the handle until the next build resets it
+    with pytest.raises(ValueError):
+        bk.grid.status()
 
 
 def test_exact_cut_radius(oracle):

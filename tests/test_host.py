@@ -98,3 +98,15 @@ def test_autotune_lookup_and_density_class(tmp_path, monkeypatch):
     assert autotune.lookup_target_ratio(100000, 16, 3) == 13.0
     assert autotune.lookup_target_ratio(12000, 16, 3) == 6.5      # nearest size
     assert autotune.lookup_target_ratio(100000, 32, 3) == DEFAULT_TARGET_RATIO
+
+
+def test_quotient32_correctly_rounded():
+    """fl32(x / fl32(size)) through float64 (double rounding is innocuous
+    for a quotient of fp32 values) equals numpy's fp32 division bitwise."""
+    from flashknn_exprepo_amd.functions.FlashKnnWrapper import quotient32
+    rng = np.random.default_rng(3)
+    x = np.concatenate([np.array([5.0399995, -7.5599995, 0.0, -0.0, 1e-40, 3e38], np.float32),
+                        (rng.standard_normal(200000) * 10).astype(np.float32)])
+    for size in (0.06, 0.04, 0.24, 1e-3, 7.0):
+        got = quotient32(torch.from_numpy(x), size).numpy()
+        assert np.array_equal(got.view(np.uint32), (x / np.float32(size)).view(np.uint32)), size

@@ -288,3 +288,45 @@ def test_adaptive_selection_rule(oracle):
         assert (s == top) or (cnt >= 16).all()
     # rows are sorted and padded with the point itself
     assert (np.diff(d, axis=1) >= 0).all()
+
+
+# ------------------------------------------------- voxel-first subsampling
+
+def test_voxel_first_hand_vectors(oracle):
+    pts = np.array([[0.01, 0, 0], [0.02, 0, 0], [0.5, 0, 0], [0.03, 0, 0]], np.float32)
+    assert oracle.voxel_first(pts, 0.1).tolist() == [0, 2]
+    # voxel coordinates are keyed by their low 16 bits (xyz2key, depth 16):
+    # x = 0.5 and 65536.5 share voxel key 0; -0.5 (v = -1) and 65535.5 share 0xffff
+    a = np.array([[0.5, 0, 0], [65536.5, 0, 0], [-0.5, 0, 0], [65535.5, 0, 0],
+                  [0.5, 1.5, 0]], np.float32)
+    assert oracle.voxel_first(a, 1.0).tolist() == [0, 2, 4]
+    # clouds are voxelised separately
+    assert oracle.voxel_first(a[:2], 1.0, offsets=[1, 2]).tolist() == [0, 1]
+
+
+def test_voxel_first_true_quotient(oracle):
+    """floor(x / size) with the correctly rounded fp32 quotient; the product
+    with the fp32 reciprocal floors higher on this value."""
+    x = np.array([[5.0399995, 0, 0], [5.04, 0, 0]], np.float32)
+    assert np.floor(x[0, 0] / np.float32(0.06)) == 83
+    assert np.floor(x[0, 0] * (np.float32(1) / np.float32(0.06))) == 84
+    assert oracle.voxel_keys(x, 0.06)[0] != oracle.voxel_keys(x, 0.06)[1]
+    assert oracle.voxel_first(x, 0.06).tolist() == [0, 1]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_voxel_first_vs_python_loop(oracle, seed):
+    rng = np.random.default_rng(seed)
+    pts = (rng.random((3000, 3), dtype=np.float32) * 4 - 2).astype(np.float32)
+    off = np.array([700, 700, 2100, 3000], np.int32)
+    for o in (None, off):
+        got = oracle.voxel_first(pts, 0.3, o)
+        assert np.array_equal(got, oracle.voxel_first_python(pts, 0.3, o))
+        # invariants: one point per voxel key; every key represented by an
+        # index no larger than any of its points
+        key = oracle.voxel_keys(pts, 0.3, o)
+        assert len(np.unique(key[got])) == len(got) == len(np.unique(key))
+        first = {}
+        for i, k in enumerate(key.tolist()):
+            first.setdefault(k, i)
+        assert sorted(first.values()) == got.tolist()
