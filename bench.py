@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="also run 5 launches of a plain fp32 copy kernel over this many "
+                         "MiB (the known-size reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; the kNN path "
@@ -150,6 +153,15 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
                 "traffic": traffic, "kernel": "k_query<16> (first pass)",
                 "kernel_ms": round(p1_ms, 5), "algorithmic_bytes": bytes_q}
+
+    if args.copy_mb > 0:
+        # torch.neg: one elementwise kernel reading and writing copy_mb MiB
+        a = torch.randn(args.copy_mb * (1 << 18), device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        del a, b
 
     extra = {"build_ms": round(build_ms, 4), "cells": ncells,
              "certified_first_pass": stats["certified_first_pass"],

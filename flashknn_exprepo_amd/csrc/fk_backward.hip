@@ -30,8 +30,8 @@ struct Workspace {
   int32_t *counts;  // [ns]
   int32_t *start;   // [ns + 1]
   int32_t *part;    // scan partials
-  int32_t *rank;    // [E]
-  int32_t *slots;   // [E]
+  int32_t *rank;    // [E] rank in the segment, then the sorted segments
+  int32_t *slots;   // [E] placed entries (segment order unsorted)
 };
 
 inline int64_t align_up(int64_t v) { return (v + 255) & ~int64_t(255); }
@@ -69,19 +69,43 @@ __global__ void k_rc_place(const int64_t *idx, int64_t E, const int32_t *start,
   if (r >= 0) slots[start[idx[e]] + r] = (int32_t)e;
 }
 
-// Insertion sort of each segment by entry id: fixes the summation order.
-__global__ void k_rc_sort(const int32_t *start, int64_t ns, int32_t *slots) {
-  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (m >= ns) return;
-  const int32_t b = start[m], e = start[m + 1];
-  for (int32_t i = b + 1; i < e; ++i) {
-    const int32_t v = slots[i];
-    int32_t j = i - 1;
-    while (j >= b && slots[j] > v) {
-      slots[j + 1] = slots[j];
-      --j;
+// Sort every segment by entry id (fixes the summation order): one wave per
+// segment; each lane takes one entry of a 64-entry chunk and counts the
+// segment's entries below it (ids are distinct, so the counts are the
+// ranks), reading the segment from LDS when it fits.
+constexpr int kRankWaves = 4;
+constexpr int kRankStage = 512;  // entries staged in LDS per wave
+
+__global__ void __launch_bounds__(64 * kRankWaves)
+k_rc_rank(const int32_t *start, int64_t ns, const int32_t *unsorted,
+          int32_t *sorted) {
+  __shared__ int32_t stage[kRankWaves * kRankStage];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t m = (int64_t)blockIdx.x * kRankWaves + w;
+  if (m >= ns) return;  // wave-uniform
+  const int32_t b = start[m], L = start[m + 1] - b;
+  const int32_t *seg = unsorted + b;
+  int32_t *st = stage + w * kRankStage;
+  const bool staged = L <= kRankStage;
+  if (staged)
+    for (int t = lane; t < L; t += 64) st[t] = seg[t];
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int32_t *src = staged ? st : seg;
+  for (int c0 = 0; c0 < L; c0 += 64) {
+    const bool have = c0 + lane < L;
+    const int32_t x = have ? src[c0 + lane] : 0;
+    int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+    int t = 0;
+    for (; t + 4 <= L; t += 4) {
+      r0 += src[t] < x;
+      r1 += src[t + 1] < x;
+      r2 += src[t + 2] < x;
+      r3 += src[t + 3] < x;
     }
-    slots[j + 1] = v;
+    for (; t < L; ++t) r0 += src[t] < x;
+    if (have) sorted[b + r0 + r1 + r2 + r3] = x;
   }
 }
 
@@ -112,7 +136,30 @@ __global__ void k_dist_bw_p(const float *q, const float *p, int64_t ns,
   if (m >= ns) return;
   const float px = p[3 * m], py = p[3 * m + 1], pz = p[3 * m + 2];
   float ax = 0.f, ay = 0.f, az = 0.f;
-  for (int32_t t = start[m]; t < start[m + 1]; ++t) {
+  const int32_t b = start[m], end = start[m + 1];
+  int32_t t = b;
+  // loads of four entries in flight; the sum still runs in entry order
+  for (; t + 4 <= end; t += 4) {
+    int32_t e[4];
+    float c[4], dx[4], dy[4], dz[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) e[u] = slots[t + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t i = e[u] / k;
+      c[u] = 2.f * g[e[u]];
+      dx[u] = q[3 * i] - px;
+      dy[u] = q[3 * i + 1] - py;
+      dz[u] = q[3 * i + 2] - pz;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      ax -= c[u] * dx[u];
+      ay -= c[u] * dy[u];
+      az -= c[u] * dz[u];
+    }
+  }
+  for (; t < end; ++t) {
     const int32_t e = slots[t];
     const int64_t i = e / k;
     const float c = 2.f * g[e];
@@ -151,31 +198,57 @@ __global__ void k_idw_bw_feat(int64_t ns, int32_t C, const float *w, int32_t k,
   const int64_t m = t / C;
   const int c = (int)(t - m * C);
   float acc = 0.f;
-  for (int32_t s = start[m]; s < start[m + 1]; ++s) {
+  const int32_t b = start[m], end = start[m + 1];
+  int32_t s = b;
+  for (; s + 4 <= end; s += 4) {  // four gathers in flight, summed in order
+    float wv[4], gv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t e = slots[s + u];
+      wv[u] = w[e];
+      gv[u] = gout[(int64_t)(e / k) * C + c];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) acc += wv[u] * gv[u];
+  }
+  for (; s < end; ++s) {
     const int32_t e = slots[s];
     acc += w[e] * gout[(int64_t)(e / k) * C + c];
   }
   gfeat[t] = acc;
 }
 
-// grad_w[i, j] = <grad_out[i, :], feat[idx[i,j], :]>  (thread per entry).
+// grad_w[i, j] = <grad_out[i, :], feat[idx[i,j], :]>: 16 lanes per entry
+// read the two rows coalesced (lane s takes channels s, s + 16, ...) and
+// reduce by a fixed butterfly, so the result is reproducible.
+constexpr int kDotLanes = 16;
+
 __global__ void k_idw_bw_w(const float *feat, int64_t ns, int32_t C,
                            const int64_t *idx, int64_t nq, int32_t k,
                            const float *gout, float *gw) {
-  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (e >= nq * k) return;
-  const int64_t m = idx[e];
+  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t e = t / kDotLanes;
+  const int sub = (int)(t % kDotLanes);
+  const bool ok = e < nq * k;
+  const int64_t m = ok ? idx[e] : -1;
   float acc = 0.f;
   if (m >= 0 && m < ns) {
     const float *f = feat + m * C;
     const float *go = gout + (e / k) * C;
-    for (int c = 0; c < C; ++c) acc += go[c] * f[c];
+    for (int c = sub; c < C; c += kDotLanes) acc += go[c] * f[c];
   }
-  gw[e] = acc;
+  // every lane of the wave takes part (entries past the end add zeros)
+#pragma unroll
+  for (int o = kDotLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kDotLanes);
+  if (ok && sub == 0) gw[e] = acc;
 }
 
+// Reverse CSR of idx: segment m = [start[m], start[m+1]) of *sorted holds the
+// entries e with idx[e] == m, ascending.
 hipError_t build_reverse_csr(const int64_t *idx, int64_t E, int64_t ns,
-                             const Workspace &w, hipStream_t s) {
+                             const Workspace &w, const int32_t **sorted,
+                             hipStream_t s) {
+  *sorted = w.rank;
   hipLaunchKernelGGL(k_zero, dim3(nblocks(ns) > 4096 ? 4096 : nblocks(ns)),
                      dim3(kBlock), 0, s, w.counts, ns);
   if (E > 0)
@@ -186,9 +259,12 @@ hipError_t build_reverse_csr(const int64_t *idx, int64_t E, int64_t ns,
   if (E > 0)
     hipLaunchKernelGGL(k_rc_place, dim3(nblocks(E)), dim3(kBlock), 0, s, idx, E,
                        w.start, w.rank, w.slots);
-  if (ns > 0)
-    hipLaunchKernelGGL(k_rc_sort, dim3(nblocks(ns)), dim3(kBlock), 0, s,
-                       w.start, ns, w.slots);
+  // sorted segments go to w.rank (free once placed); w.slots then points there
+  if (ns > 0 && E > 0) {
+    const int64_t nb = (ns + kRankWaves - 1) / kRankWaves;
+    hipLaunchKernelGGL(k_rc_rank, dim3((unsigned)nb), dim3(64 * kRankWaves), 0, s,
+                       w.start, ns, w.slots, w.rank);
+  }
   return hipGetLastError();
 }
 
@@ -211,10 +287,11 @@ hipError_t launch_dist_backward(const float *q, int64_t nq, const float *p,
                        p, ns, idx, k, grad_d, grad_q);
   if (grad_p && ns > 0) {
     const Workspace w = carve(ws, ns, E);
-    hipError_t err = build_reverse_csr(idx, E, ns, w, s);
+    const int32_t *sorted = nullptr;
+    hipError_t err = build_reverse_csr(idx, E, ns, w, &sorted, s);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_dist_bw_p, dim3(nblocks(ns)), dim3(kBlock), 0, s, q, p,
-                       ns, k, grad_d, w.start, w.slots, grad_p);
+                       ns, k, grad_d, w.start, sorted, grad_p);
   }
   return hipGetLastError();
 }
@@ -236,14 +313,15 @@ hipError_t launch_idw_backward(const float *feat, int64_t ns, int32_t C,
                                hipStream_t s) {
   const int64_t E = nq * k;
   if (grad_w && E > 0)
-    hipLaunchKernelGGL(k_idw_bw_w, dim3(nblocks(E)), dim3(kBlock), 0, s, feat, ns,
-                       C, idx, nq, k, grad_out, grad_w);
+    hipLaunchKernelGGL(k_idw_bw_w, dim3(nblocks(E * kDotLanes)), dim3(kBlock), 0, s,
+                       feat, ns, C, idx, nq, k, grad_out, grad_w);
   if (grad_feat && ns * C > 0) {
     const Workspace wsp = carve(ws, ns, E);
-    hipError_t err = build_reverse_csr(idx, E, ns, wsp, s);
+    const int32_t *sorted = nullptr;
+    hipError_t err = build_reverse_csr(idx, E, ns, wsp, &sorted, s);
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * C)), dim3(kBlock), 0, s,
-                       ns, C, w, k, grad_out, wsp.start, wsp.slots, grad_feat);
+                       ns, C, w, k, grad_out, wsp.start, sorted, grad_feat);
   }
   return hipGetLastError();
 }

@@ -211,24 +211,34 @@ struct TopK {
 
 // Scan support records [b, e) for this lane's query.  Wave-uniform trip
 // count (the longest lane); a merge runs whenever any lane could overflow.
+//
+// The rounds that only append (the inner loop) never touch the register
+// list: it changes only in the flush between them, so the compiler keeps it
+// in place instead of copying it around every round.
 template <int KC>
 __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, float qx,
                                            float qy, float qz,
                                            const float4 *__restrict__ srec) {
+  constexpr int kRoom = buffer_slots<KC>() - kUnroll;
   const int len = e - b;
-  for (int s = 0; wave_any(s < len); s += kUnroll) {
-    if (wave_any(t.cnt > buffer_slots<KC>() - kUnroll)) t.flush();
-    float4 r[kUnroll];
+  int s = 0;
+  for (;;) {
+    while (wave_any(s < len) && !wave_any(t.cnt > kRoom)) {
+      float4 r[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const bool ok = s + u < len;
-      r[u] = srec[ok ? b + s + u : 0];
-    }
+      for (int u = 0; u < kUnroll; ++u) {
+        const bool ok = s + u < len;
+        r[u] = srec[ok ? b + s + u : 0];
+      }
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const float d = sqdist(r[u].x, r[u].y, r[u].z, qx, qy, qz);
-      t.push(make_key(d, __float_as_uint(r[u].w)), s + u < len);
+      for (int u = 0; u < kUnroll; ++u) {
+        const float d = sqdist(r[u].x, r[u].y, r[u].z, qx, qy, qz);
+        t.push(make_key(d, __float_as_uint(r[u].w)), s + u < len);
+      }
+      s += kUnroll;
     }
+    if (!wave_any(s < len)) break;
+    t.flush();
   }
 }
 
