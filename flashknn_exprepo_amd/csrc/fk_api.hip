@@ -527,7 +527,8 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.nq = (int32_t)(self ? g->n_points : g->n_queries);
   a.k = p->k;
   a.R = p->radius_cells;
-  a.cut_r2 = p->cut_r2;
+  // d < cut with cut <= 0 holds for no candidate: a cut of +0 says the same
+  a.cut_r2 = p->cut_r2 > 0.f ? p->cut_r2 : 0.f;
   a.pad_mode = p->pad_mode;
   a.dist_mode = p->dist_mode;
   a.pad_dist = p->pad_dist;

@@ -10,15 +10,20 @@
 // smaller support id (the reference keeps the first one scanned).
 //
 // Mapping (designed for 64-lane wavefronts, not a translation of the
-// reference's lane groups): one lane owns one query; a wave owns 64
-// consecutive queries of the cell-sorted order, so its lanes walk the same or
-// adjacent column segments and their candidate loads coalesce or broadcast.
-// Each lane keeps its best KC keys sorted in VGPRs as u64 (d bits << 32 | id):
-// for d >= 0 the u64 order is the (d, id) order, so one 64-bit compare
-// decides both the distance and the tie.  Accepted candidates are appended
-// to a per-lane LDS buffer (one ds_write, no divergent insertion); when any
-// lane's buffer is nearly full the wave runs one bitonic sort of the buffer
-// and a bitonic merge into the register list.
+// reference's lane groups): a group of G lanes (1, 2 or 4 by K class) owns
+// one query and a wave owns 64 / G consecutive queries of the cell-sorted
+// order, so its lanes walk the same or adjacent column segments and their
+// candidate loads coalesce or broadcast; lane g of a group takes every G-th
+// record of each column.  Each lane keeps its best KC keys sorted in VGPRs as
+// u64 (d bits << 32 | id): for d >= 0 the u64 order is the (d, id) order, so
+// one 64-bit compare decides both the distance and the tie.  Accepted
+// candidates are appended to a per-lane LDS buffer (one ds_write, no
+// divergent insertion); when any lane's buffer is nearly full the wave runs
+// one bitonic sort of the buffer and a bitonic merge into the register
+// list, and the group adopts its smallest acceptance threshold.  At the end
+// the group's lists are merged by a butterfly over LDS.  More lanes per
+// query shorten each wave's chain of dependent loads, which bounds the pass
+// (a 100k-point cloud is only 1,536 waves of 64 queries).
 //
 // FK_MODE_EXACT adds a certification per query: if the K-th distance is
 // below the distance from the query to the nearest face of its scanned
@@ -59,16 +64,42 @@ namespace {
 #ifndef FK_WPE_64
 #define FK_WPE_64 1
 #endif
+//   FK_G_<KC>        lanes per query in the first pass (1, 2 or 4)
+#ifndef FK_G_8
+#define FK_G_8 2
+#endif
+#ifndef FK_G_16
+#define FK_G_16 2
+#endif
+#ifndef FK_G_32
+#define FK_G_32 2
+#endif
+#ifndef FK_G_64
+#define FK_G_64 1
+#endif
 
 constexpr int kUnroll = FK_UNROLL;
-constexpr uint64_t kMaxKey = ~0ull;
+// Keys: ((d bits << 32) | id) + 2^52.  For d >= 0 the u64 order is the
+// (d, id) order, and with the bias every key is a positive normal double
+// whose double order is the same, so compare-exchanges use v_min_f64 /
+// v_max_f64 (which return an input exactly) instead of a 64-bit compare and
+// four selects.  kMaxKey is +inf as a double, above every key.
+constexpr uint64_t kKeyBias = 1ull << 52;
+constexpr uint64_t kMaxKey = 0x7ff0000000000000ull;
 
 template <int KC>
 constexpr int buffer_slots() {
-  return ((KC == 16 && (FK_HALF_MASK & 1)) || (KC == 32 && (FK_HALF_MASK & 2)) ||
-          (KC == 64 && (FK_HALF_MASK & 4)))
+  // K class 8 buffers 16 keys so that it can take the 8-stream rounds
+  return KC == 8 ? 16
+         : ((KC == 16 && (FK_HALF_MASK & 1)) || (KC == 32 && (FK_HALF_MASK & 2)) ||
+            (KC == 64 && (FK_HALF_MASK & 4)))
              ? KC / 2
              : KC;
+}
+
+template <int KC>
+constexpr int lanes_per_query() {
+  return KC == 8 ? FK_G_8 : KC == 16 ? FK_G_16 : KC == 32 ? FK_G_32 : FK_G_64;
 }
 
 template <int KC>
@@ -88,15 +119,42 @@ __device__ __forceinline__ float sqdist(float px, float py, float pz, float qx,
                    __fmul_rn(dz, dz));
 }
 
+__device__ __forceinline__ uint64_t key_of_bits(uint32_t dbits, uint32_t id) {
+  return (((uint64_t)dbits << 32) | id) + kKeyBias;
+}
+
 __device__ __forceinline__ uint64_t make_key(float d, uint32_t id) {
-  return ((uint64_t)__float_as_uint(d) << 32) | id;
+  return key_of_bits(__float_as_uint(d), id);
+}
+
+__device__ __forceinline__ uint32_t key_dbits(uint64_t key) {
+  return (uint32_t)((key - kKeyBias) >> 32);
+}
+
+__device__ __forceinline__ uint32_t key_id(uint64_t key) { return (uint32_t)key; }
+
+// Keys are never NaN, so the min/max are issued directly (fmin/fmax would
+// add a NaN-quieting v_max_f64 per input in IEEE mode).
+__device__ __forceinline__ uint64_t kmin(uint64_t a, uint64_t b) {
+  double r;
+  asm("v_min_f64 %0, %1, %2"
+      : "=v"(r)
+      : "v"(__longlong_as_double((long long)a)), "v"(__longlong_as_double((long long)b)));
+  return (uint64_t)__double_as_longlong(r);
+}
+
+__device__ __forceinline__ uint64_t kmax(uint64_t a, uint64_t b) {
+  double r;
+  asm("v_max_f64 %0, %1, %2"
+      : "=v"(r)
+      : "v"(__longlong_as_double((long long)a)), "v"(__longlong_as_double((long long)b)));
+  return (uint64_t)__double_as_longlong(r);
 }
 
 __device__ __forceinline__ void cswap(uint64_t &a, uint64_t &b, bool asc) {
-  const uint64_t x = a, y = b;
-  const bool sw = asc ? (x > y) : (x < y);
-  a = sw ? y : x;
-  b = sw ? x : y;
+  const uint64_t lo = kmin(a, b), hi = kmax(a, b);
+  a = asc ? lo : hi;
+  b = asc ? hi : lo;
 }
 
 // Ascending bitonic sort of N (power of two) register keys.
@@ -152,7 +210,7 @@ struct TopK {
     k = k_;
     base = KC - k_;
     cutkey = make_key(cut_r2, 0u);
-    const uint64_t padkey = ((uint64_t)kInfBits << 32) | pad;
+    const uint64_t padkey = key_of_bits(kInfBits, pad);
 #pragma unroll
     for (int i = 0; i < KC; ++i) top[i] = i < base ? 0ull : padkey;
     thr = cutkey;  // (inf, pad) >= cutkey for every cut
@@ -172,13 +230,11 @@ struct TopK {
       b[i] = i < cnt ? v : kMaxKey;
     }
     bitonic_sort<B>(b);
-    if constexpr (B == KC) {
-      // min(top[i], b[KC-1-i]) is bitonic and holds the KC smallest
+    if constexpr (B >= KC) {
+      // min(top[i], b[KC-1-i]) is bitonic and holds the KC smallest (only
+      // the KC smallest buffered keys can enter)
 #pragma unroll
-      for (int i = 0; i < KC; ++i) {
-        const uint64_t o = b[KC - 1 - i];
-        top[i] = o < top[i] ? o : top[i];
-      }
+      for (int i = 0; i < KC; ++i) top[i] = kmin(top[i], b[KC - 1 - i]);
       bitonic_merge<KC>(top);
     } else {
       // B = KC/2: the half-cleaner min(top[B+i], b[B-1-i]) gives the B
@@ -186,10 +242,7 @@ struct TopK {
       // merge the two sorted halves (second half read reversed = bitonic).
       uint64_t h[B];
 #pragma unroll
-      for (int i = 0; i < B; ++i) {
-        const uint64_t o = b[B - 1 - i];
-        h[i] = o < top[B + i] ? o : top[B + i];
-      }
+      for (int i = 0; i < B; ++i) h[i] = kmin(top[B + i], b[B - 1 - i]);
       bitonic_merge<B>(h);
 #pragma unroll
       for (int i = 0; i < B; ++i) top[KC - 1 - i] = h[i];
@@ -207,6 +260,48 @@ struct TopK {
   }
 
   __device__ __forceinline__ uint64_t kth() const { return top[KC - 1]; }
+
+  // G lanes scan disjoint parts of one query's candidates.  The k-th key of
+  // their union is <= each lane's k-th, so every lane may reject against the
+  // smallest threshold of its group.
+  template <int G>
+  __device__ __forceinline__ void share_threshold() {
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+      const uint64_t u = __shfl_xor(thr, o, kWave);
+      thr = u < thr ? u : thr;
+    }
+  }
+
+  // Butterfly merge of the group's partial lists: afterwards every lane of
+  // the group holds the k best keys of the union.  Each round a lane copies
+  // its k real keys to slots 0..k-1 of its (empty) LDS buffer and reads its
+  // partner's; min(top[i], partner[KC-1-i]) is bitonic over the real slots
+  // (the zero slots below `base` meet +inf), and one bitonic merge sorts it.
+  template <int G>
+  __device__ __forceinline__ void group_merge(int lane) {
+    static_assert(G == 1 || buffer_slots<KC>() >= KC, "merge needs k buffer slots");
+#pragma unroll
+    for (int o = 1; o < G; o <<= 1) {
+#pragma unroll
+      for (int j = 0; j < KC; ++j)
+        if (j >= base) lb[(j - base) * kWave] = top[j];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const uint64_t *pb = lb + ((lane ^ o) - lane);
+      uint64_t p[KC];
+#pragma unroll
+      for (int i = 0; i < KC; ++i) p[i] = i < k ? pb[i * kWave] : kMaxKey;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int i = 0; i < KC; ++i) top[i] = kmin(top[i], p[KC - 1 - i]);
+      bitonic_merge<KC>(top);
+    }
+    refresh_thr();
+  }
 };
 
 // Scan support records [b, e) for this lane's query.  Wave-uniform trip
@@ -215,12 +310,17 @@ struct TopK {
 // The rounds that only append (the inner loop) never touch the register
 // list: it changes only in the flush between them, so the compiler keeps it
 // in place instead of copying it around every round.
-template <int KC>
-__device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, float qx,
-                                           float qy, float qz,
+//
+// With G lanes per query, lane `sub` of the group takes records
+// b + sub, b + sub + G, ... (the group's loads are adjacent).
+template <int KC, int G>
+__device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, int sub,
+                                           float qx, float qy, float qz,
                                            const float4 *__restrict__ srec) {
   constexpr int kRoom = buffer_slots<KC>() - kUnroll;
-  const int len = e - b;
+  const int span = e - b - sub;
+  const int len = span > 0 ? (span + G - 1) / G : 0;  // this lane's records
+  const int first = b + sub;
   int s = 0;
   for (;;) {
     while (wave_any(s < len) && !wave_any(t.cnt > kRoom)) {
@@ -228,7 +328,7 @@ __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, float qx,
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const bool ok = s + u < len;
-        r[u] = srec[ok ? b + s + u : 0];
+        r[u] = srec[ok ? first + (s + u) * G : 0];
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
@@ -239,6 +339,7 @@ __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, float qx,
     }
     if (!wave_any(s < len)) break;
     t.flush();
+    t.template share_threshold<G>();
   }
 }
 
@@ -262,15 +363,71 @@ __device__ __forceinline__ void column_range(const CloudInfo &ci,
 // the k-th distance (the acceptance threshold) is tight early and fewer
 // later candidates get buffered and merged.  Then the others in raster
 // order (dx, dy) = (0, 0) is skipped.
-template <int KC>
+// The 8 columns around the query's own (R = 1) as parallel streams: their
+// cell_start reads are issued together, then every round loads the next
+// record of each column (lane `sub` of a group takes every G-th), so the
+// rounds follow the longest column instead of the sum of all eight.
+template <int KC, int G>
+__device__ __forceinline__ void scan_ring8(TopK<KC> &t, const QueryArgs &a,
+                                           const CloudInfo &ci, int4 qc,
+                                           float4 qr, bool active, int sub) {
+  constexpr int kRoom = buffer_slots<KC>() - 8;
+  static_assert(kRoom >= 0, "buffer too small for 8 streams");
+  int p[8], e[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const int j = c < 4 ? c : c + 1;  // 3x3 raster order without the centre
+    int b = 0, en = 0;
+    if (active)
+      column_range(ci, a.cell_start, qc.x + j / 3 - 1, qc.y + j % 3 - 1, qc.z - 1,
+                   qc.z + 1, b, en);
+    p[c] = b + sub;
+    e[c] = en;
+  }
+  auto left = [&]() {
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) any |= p[c] < e[c];
+    return wave_any(any);
+  };
+  for (;;) {
+    while (left() && !wave_any(t.cnt > kRoom)) {
+      float4 r[8];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) r[c] = a.srec[p[c] < e[c] ? p[c] : 0];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const float d = sqdist(r[c].x, r[c].y, r[c].z, qr.x, qr.y, qr.z);
+        t.push(make_key(d, __float_as_uint(r[c].w)), p[c] < e[c]);
+        p[c] += G;
+      }
+    }
+    if (!left()) break;
+    t.flush();
+    t.template share_threshold<G>();
+  }
+}
+
+template <int KC, int G>
 __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
-                                           float4 qr, int R, bool active) {
+                                           float4 qr, int R, bool active,
+                                           int sub) {
+  if constexpr (KC <= 16 && buffer_slots<KC>() >= 16) {
+    if (R == 1) {
+      int b = 0, e = 0;
+      if (active)
+        column_range(ci, a.cell_start, qc.x, qc.y, qc.z - 1, qc.z + 1, b, e);
+      scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
+      scan_ring8<KC, G>(t, a, ci, qc, qr, active, sub);
+      return;
+    }
+  }
   {
     int b = 0, e = 0;
     if (active)
       column_range(ci, a.cell_start, qc.x, qc.y, qc.z - R, qc.z + R, b, e);
-    scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
+    scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
   }
   for (int dx = -R; dx <= R; ++dx) {
     for (int dy = -R; dy <= R; ++dy) {
@@ -279,7 +436,7 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
       if (active)
         column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - R,
                      qc.z + R, b, e);
-      scan_range<KC>(t, b, e, qr.x, qr.y, qr.z, a.srec);
+      scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
     }
   }
 }
@@ -370,31 +527,34 @@ __device__ __forceinline__ int64_t out_base(const QueryArgs &a,
   return (int64_t)row * k;
 }
 
-template <int KC>
+// One output entry (index + distance) from a key.
+__device__ __forceinline__ void write_entry(const QueryArgs &a, int64_t o,
+                                            uint64_t key) {
+  const int id = (int)key_id(key);
+  if (a.idx64)
+    ((int64_t *)a.out_idx)[o] = (int64_t)id;
+  else
+    ((int32_t *)a.out_idx)[o] = id;
+  if (a.out_d) {
+    const uint32_t db = key_dbits(key);
+    float d = __uint_as_float(db);
+    if (db == kInfBits)
+      d = a.pad_dist;
+    else if (a.dist_mode == FK_DIST_SQRT)
+      d = sqrtf(d);
+    a.out_d[o] = d;
+  }
+}
+
+template <int KC, int G>
 __device__ __forceinline__ void write_row(const QueryArgs &a, const TopK<KC> &t,
-                                          int64_t o) {
-  // real entry j lives in register slot base + j
+                                          int64_t o, int sub) {
+  // real entry j lives in register slot base + j; lane `sub` of the group
+  // writes the entries j with j % G == sub
   const int64_t ob = o - t.base;
 #pragma unroll
-  for (int j = 0; j < KC; ++j) {
-    if (j >= t.base) {
-      const uint64_t key = t.top[j];
-      const int id = (int)(uint32_t)key;
-      if (a.idx64)
-        ((int64_t *)a.out_idx)[ob + j] = (int64_t)id;
-      else
-        ((int32_t *)a.out_idx)[ob + j] = id;
-      if (a.out_d) {
-        const uint32_t db = (uint32_t)(key >> 32);
-        float d = __uint_as_float(db);
-        if (db == kInfBits)
-          d = a.pad_dist;
-        else if (a.dist_mode == FK_DIST_SQRT)
-          d = sqrtf(d);
-        a.out_d[ob + j] = d;
-      }
-    }
-  }
+  for (int j = 0; j < KC; ++j)
+    if (j >= t.base && ((j - t.base) & (G - 1)) == sub) write_entry(a, ob + j, t.top[j]);
 }
 
 __device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
@@ -410,13 +570,15 @@ __device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
   return base + __popcll(below);
 }
 
-// One wave: queries [q0, q0 + 64) of the sorted query list, all lanes of one
-// K class.  Used by the plain kernel and by the persistent batched kernel.
-template <int KC>
+// One wave: queries [q0, q0 + 64 / G) of the sorted query list, G lanes per
+// query, all of one K class.  Used by the plain kernel and by the persistent
+// batched kernel.
+template <int KC, int G>
 __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
                                          uint64_t *lane_buf) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int qi = q0 + lane;
+  const int sub = lane & (G - 1);
+  const int qi = q0 + lane / G;
   const bool active = qi < qend;
   const int qs = active ? qi : (qend - 1);
   const float4 qr = a.qrec[qs];
@@ -425,34 +587,38 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
   const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
   TopK<KC> t;
   t.init(lane_buf, k, a.cut_r2, pad_value(a, qr, qs));
-  scan_block<KC>(t, a, ci, qc, qr, a.R, active);
+  scan_block<KC, G>(t, a, ci, qc, qr, a.R, active, sub);
   if (wave_any(t.cnt > 0)) t.flush();
+  if constexpr (G > 1) t.template group_merge<G>(lane);
 
   bool done = true;
   if (a.exact) {
     double umax;
     const double gap = cert_gap(ci, qc, qr, a.R, umax);
-    const uint64_t kk = t.kth();
-    float eff = __uint_as_float((uint32_t)(kk >> 32));
+    float eff = __uint_as_float(key_dbits(t.kth()));
     eff = eff < a.cut_r2 ? eff : a.cut_r2;
     done = certified(gap, eff, ci.inv_cell, umax, a.R);
-    const int nc = __popcll(__ballot(active && done));
+    // one count and one pending slot per query: its group's lane 0
+    const bool lead = sub == 0;
+    const int nc = __popcll(__ballot(active && done && lead));
     if (lane == 0 && nc) atomicAdd(a.scalars + kStatCertified, nc);
-    const int slot = wave_prefix_slot(active && !done, a.scalars + kPending);
+    int slot = wave_prefix_slot(active && !done && lead, a.scalars + kPending);
+    if constexpr (G > 1) slot = __shfl(slot, lane & ~(G - 1), kWave);
     if (active && !done && slot >= a.nq) {
       // the pending counter was not reset before this pass: refuse to write
-      raise_status(a.scalars, FK_ERR_STATE);
+      if (lead) raise_status(a.scalars, FK_ERR_STATE);
     } else if (active && !done) {
-      a.pend_q[slot] = qs;
+      if (lead) a.pend_q[slot] = qs;
       const int64_t pb = (int64_t)slot * a.k - t.base;
 #pragma unroll
       for (int j = 0; j < KC; ++j)
-        if (j >= t.base) a.pend_keys[pb + j] = t.top[j];
+        if (j >= t.base && ((j - t.base) & (G - 1)) == sub)
+          a.pend_keys[pb + j] = t.top[j];
     }
   }
   if (active && done) {
     const int row = __float_as_int(qr.w);
-    write_row<KC>(a, t, out_base(a, ci, qc.w, row, k));
+    write_row<KC, G>(a, t, out_base(a, ci, qc.w, row, k), sub);
   }
 }
 
@@ -469,11 +635,12 @@ k_query(QueryArgs a) {
   __shared__ uint64_t sbuf[WPB * BS * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  const int q0 = (blk * WPB + w) * kWave;
+  constexpr int G = lanes_per_query<KC>();
+  const int q0 = (blk * WPB + w) * (kWave / G);
   // wave-uniform exits: past the end, or the build/attach raised an error
   // (its tables may then be incomplete)
   if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
-  run_tile<KC>(a, q0, a.nq, sbuf + w * BS * kWave + lane);
+  run_tile<KC, G>(a, q0, a.nq, sbuf + w * BS * kWave + lane);
 }
 
 // ------------------------------------------------- cooperative expansion
@@ -490,8 +657,6 @@ k_query(QueryArgs a) {
 constexpr int kCoopWaves = 4;          // waves per block
 constexpr int kCoopCap = 256;          // buffered keys per wave
 
-__device__ __forceinline__ uint64_t umin64(uint64_t a, uint64_t b) { return a < b ? a : b; }
-__device__ __forceinline__ uint64_t umax64(uint64_t a, uint64_t b) { return a < b ? b : a; }
 
 // Ascending bitonic sort of one key per lane across the wave.
 __device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
@@ -502,7 +667,7 @@ __device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
       const uint64_t o = __shfl_xor(v, j, kWave);
       const bool up = (lane & k2) == 0;
       const bool lower = (lane & j) == 0;
-      v = (lower == up) ? umin64(v, o) : umax64(v, o);
+      v = (lower == up) ? kmin(v, o) : kmax(v, o);
     }
   }
   return v;
@@ -510,11 +675,11 @@ __device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
 
 // L and S ascending across lanes -> the 64 smallest of both, ascending.
 __device__ __forceinline__ uint64_t wave_merge(uint64_t L, uint64_t S, int lane) {
-  uint64_t v = umin64(L, __shfl(S, kWave - 1 - lane, kWave));
+  uint64_t v = kmin(L, __shfl(S, kWave - 1 - lane, kWave));
 #pragma unroll
   for (int j = kWave >> 1; j > 0; j >>= 1) {
     const uint64_t o = __shfl_xor(v, j, kWave);
-    v = (lane & j) == 0 ? umin64(v, o) : umax64(v, o);
+    v = (lane & j) == 0 ? kmin(v, o) : kmax(v, o);
   }
   return v;
 }
@@ -524,7 +689,7 @@ __device__ __forceinline__ uint64_t wave_bitonic_merge(uint64_t v, int lane) {
 #pragma unroll
   for (int j = kWave >> 1; j > 0; j >>= 1) {
     const uint64_t o = __shfl_xor(v, j, kWave);
-    v = (lane & j) == 0 ? umin64(v, o) : umax64(v, o);
+    v = (lane & j) == 0 ? kmin(v, o) : kmax(v, o);
   }
   return v;
 }
@@ -562,8 +727,8 @@ struct CoopList {
         // with +inf to 128): C[i] = min(L[i], S'[127 - i]) is bitonic with
         // C[0..63] = L[0..63]; the first merge stage (distance 64) is the
         // in-lane min/max, then each half merges across the wave.
-        const uint64_t c1 = umin64(L[1], __shfl(S, kWave - 1 - lane, kWave));
-        const uint64_t lo = umin64(L[0], c1), hi = umax64(L[0], c1);
+        const uint64_t c1 = kmin(L[1], __shfl(S, kWave - 1 - lane, kWave));
+        const uint64_t lo = kmin(L[0], c1), hi = kmax(L[0], c1);
         L[0] = wave_bitonic_merge(lo, lane);
         L[1] = wave_bitonic_merge(hi, lane);
       }
@@ -637,29 +802,10 @@ __device__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &
   if (t.cnt > 0) t.flush(lane);
 }
 
-// One output entry (index + distance) from a key.
-__device__ __forceinline__ void write_entry(const QueryArgs &a, int64_t o,
-                                            uint64_t key) {
-  const int id = (int)(uint32_t)key;
-  if (a.idx64)
-    ((int64_t *)a.out_idx)[o] = (int64_t)id;
-  else
-    ((int32_t *)a.out_idx)[o] = id;
-  if (a.out_d) {
-    const uint32_t db = (uint32_t)(key >> 32);
-    float d = __uint_as_float(db);
-    if (db == kInfBits)
-      d = a.pad_dist;
-    else if (a.dist_mode == FK_DIST_SQRT)
-      d = sqrtf(d);
-    a.out_d[o] = d;
-  }
-}
-
 // min(k-th distance, cut) of a cooperative list.
 template <int P>
 __device__ __forceinline__ float coop_eff(const CoopList<P> &t, float cut_r2) {
-  const float e = __uint_as_float((uint32_t)(t.kth() >> 32));
+  const float e = __uint_as_float(key_dbits(t.kth()));
   return e < cut_r2 ? e : cut_r2;
 }
 
@@ -733,7 +879,7 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
     t.cnt = 0;
     t.k = k;
     t.cutkey = make_key(a.cut_r2, 0u);
-    const uint64_t padkey = ((uint64_t)kInfBits << 32) | pad_value(a, qr, qs);
+    const uint64_t padkey = key_of_bits(kInfBits, pad_value(a, qr, qs));
     t.L[0] = lane < k ? padkey : kMaxKey;
     t.L[1] = lane + kWave < k ? padkey : kMaxKey;
     t.refresh(lane);
@@ -764,13 +910,24 @@ __device__ __forceinline__ int item_to_cloud(const int32_t *prefix, int B,
   return lo;
 }
 
+// Largest LDS buffer of the K classes up to MAXKC.
+template <int MAXKC>
+constexpr int batched_slots() {
+  int m = buffer_slots<8>();
+  if (MAXKC >= 16 && buffer_slots<16>() > m) m = buffer_slots<16>();
+  if (MAXKC >= 32 && buffer_slots<32>() > m) m = buffer_slots<32>();
+  if (MAXKC >= 64 && buffer_slots<64>() > m) m = buffer_slots<64>();
+  return m;
+}
+
 template <int MAXKC>
 __global__ void __launch_bounds__(64 * waves_per_block<MAXKC>())
 k_batched(QueryArgs a) {
   constexpr int WPB = waves_per_block<MAXKC>();
-  __shared__ uint64_t sbuf[WPB * MAXKC * kWave];
+  constexpr int BS = batched_slots<MAXKC>();
+  __shared__ uint64_t sbuf[WPB * BS * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  uint64_t *lane_buf = sbuf + w * MAXKC * kWave + lane;
+  uint64_t *lane_buf = sbuf + w * BS * kWave + lane;
   const int32_t *prefix = a.item_cloud;
   const int nitems = a.scalars[kStatus] != 0 ? 0 : prefix[a.num_clouds];
   for (;;) {
@@ -790,13 +947,13 @@ k_batched(QueryArgs a) {
       continue;
     }
     if (k <= 8 || MAXKC == 8)
-      run_tile<8>(a, q0, qe, lane_buf);
+      run_tile<8, 1>(a, q0, qe, lane_buf);
     else if (k <= 16 || MAXKC == 16)
-      run_tile<(MAXKC < 16 ? MAXKC : 16)>(a, q0, qe, lane_buf);
+      run_tile<(MAXKC < 16 ? MAXKC : 16), 1>(a, q0, qe, lane_buf);
     else if (k <= 32 || MAXKC == 32)
-      run_tile<(MAXKC < 32 ? MAXKC : 32)>(a, q0, qe, lane_buf);
+      run_tile<(MAXKC < 32 ? MAXKC : 32), 1>(a, q0, qe, lane_buf);
     else
-      run_tile<MAXKC>(a, q0, qe, lane_buf);
+      run_tile<MAXKC, 1>(a, q0, qe, lane_buf);
   }
 }
 
@@ -957,7 +1114,8 @@ unsigned expand_blocks(int nq) {
 template <int KC>
 hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
   constexpr int WPB = waves_per_block<KC>();
-  const int64_t waves = ((int64_t)a.nq + kWave - 1) / kWave;
+  constexpr int QPW = kWave / lanes_per_query<KC>();  // queries per wave
+  const int64_t waves = ((int64_t)a.nq + QPW - 1) / QPW;
   const unsigned nb = (unsigned)((waves + WPB - 1) / WPB);
   if (a.passes & 2)
     hipLaunchKernelGGL(k_query<KC>, dim3(nb), dim3(64 * WPB), 0, s, a);
