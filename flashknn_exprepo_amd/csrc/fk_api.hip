@@ -108,6 +108,10 @@ struct fk_grid {
   uint64_t *pend_keys = nullptr;
   // host state of the last build
   int built = 0, from_xyz = 0, num_down = 0, reserved = 0;
+  // query counter sets: the set of the last plain first pass, which sets are
+  // known to be zero, and the set fk_query_stats reports
+  int qset = 0, stat_set = 0;
+  bool qclean[2] = {true, true};
   int64_t n_points = 0, n_cells = -1, n_queries = -1;
   int32_t n_clouds = 0;
   // captured batched step
@@ -372,6 +376,7 @@ int fk_grid_build_cells(fk_grid *g, const float *xyz, const int64_t *grid_coord,
     if (!g->counts) FK_CHECK(ensure_cells(g, 1024));
   }
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, g->scalars);
+  g->qclean[0] = g->qclean[1] = true;
   FK_HIP(launch_build_cells(build_buffers(g), xyz, grid_coord, batch_idx, n,
                             num_clouds, num_down, s));
   g->from_xyz = 0;
@@ -401,6 +406,7 @@ int fk_grid_build_xyz(fk_grid *g, const float *xyz, const int32_t *offsets,
     if (!g->counts) FK_CHECK(ensure_cells(g, 1024));
   }
   hipLaunchKernelGGL(k_reset_scalars, dim3(1), dim3(64), 0, s, g->scalars);
+  g->qclean[0] = g->qclean[1] = true;
   FK_HIP(launch_build_xyz_cells(build_buffers(g), xyz, offsets, num_clouds, n,
                                 inv_cell, s));
   g->from_xyz = 1;
@@ -538,6 +544,8 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.exact = p->mode == FK_MODE_EXACT;
   a.passes = p->passes ? (p->passes & 7) : 7;
   a.scalars = g->scalars;
+  a.qcnt = g->scalars + kCounters + 3 * g->qset;
+  a.qcnt_next = nullptr;
   return FK_OK;
 }
 
@@ -552,7 +560,23 @@ int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
     a.pend_q = g->pend_q;
     a.pend_keys = g->pend_keys;
   }
+  // A first pass starts a new counter set (alternating); it needs a reset
+  // launch only when that set is not known to be zero (the previous first
+  // pass clears it).  An explicit bit 0 in params->passes always resets.
+  const bool first = a.passes & 2;
+  const int set = first ? (g->qset ^ 1) : g->qset;
+  const bool reset = (p->passes && (a.passes & 1)) || (first && !g->qclean[set]);
+  a.qcnt = g->scalars + kCounters + 3 * set;
+  a.qcnt_next = first && a.nq > 0 ? g->scalars + kCounters + 3 * (set ^ 1) : nullptr;
+  a.passes = (a.passes & ~1) | (reset ? 1 : 0);
   FK_HIP(launch_query(a, S(stream)));
+  g->qset = set;
+  g->stat_set = set;
+  if (reset) g->qclean[set] = true;
+  if (first) {
+    g->qclean[set] = false;
+    if (a.qcnt_next) g->qclean[set ^ 1] = true;
+  }
   return ok();
 }
 
@@ -560,8 +584,9 @@ int fk_query_stats(fk_grid *g, int64_t *first_pass_certified, int64_t *expanded,
                    void *stream) {
   FK_CHECK(check_stream_device(g));
   FK_CHECK(read_scalars(g, S(stream)));
-  if (first_pass_certified) *first_pass_certified = g->h_pinned[kStatCertified];
-  if (expanded) *expanded = g->h_pinned[kStatExpanded];
+  const int32_t *c = g->h_pinned + kCounters + 3 * g->stat_set;
+  if (first_pass_certified) *first_pass_certified = c[kCntCertified];
+  if (expanded) *expanded = c[kCntExpanded];
   return ok();
 }
 
@@ -695,6 +720,8 @@ static int batched_args(fk_grid *g, const fk_query_params *p,
   }
   a.k_per_cloud = k_per_cloud;
   a.out_offsets = out_offsets;
+  a.qcnt = g->scalars + kCounters + 3 * kBatchedSet;  // cleared by k_items
+  a.qcnt_next = nullptr;
   a.num_clouds = g->n_clouds;
   a.item_cloud = g->item_prefix;
   return FK_OK;
@@ -707,9 +734,10 @@ int fk_batched_query(fk_grid *g, const fk_query_params *p,
   QueryArgs a;
   FK_CHECK(batched_args(g, p, k_per_cloud, out_offsets, out_idx, out_dist, a));
   hipStream_t s = S(stream);
-  FK_HIP(launch_items(g->clouds, g->cell_start, g->n_clouds, g->item_prefix,
-                      g->scalars, s));
+  FK_HIP(launch_items(g->clouds, g->cell_start, g->n_clouds, p->k, g->item_prefix,
+                      g->scalars, a.qcnt, s));
   FK_HIP(launch_batched(a, p->k, s));
+  g->stat_set = kBatchedSet;
   return ok();
 }
 
@@ -731,8 +759,8 @@ int fk_batched_capture(fk_grid *g, const fk_query_params *p,
   }
   hipStream_t s = S(stream);
   FK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e1 = launch_items(g->clouds, g->cell_start, g->n_clouds,
-                               g->item_prefix, g->scalars, s);
+  hipError_t e1 = launch_items(g->clouds, g->cell_start, g->n_clouds, p->k,
+                               g->item_prefix, g->scalars, a.qcnt, s);
   hipError_t e2 = e1 == hipSuccess ? launch_batched(a, p->k, s) : e1;
   hipGraph_t graph = nullptr;
   hipError_t e3 = hipStreamEndCapture(s, &graph);
@@ -749,6 +777,7 @@ int fk_batched_capture(fk_grid *g, const fk_query_params *p,
 int fk_graph_replay(fk_grid *g, void *stream) {
   if (!g || !g->gexec) return fail(FK_ERR_STATE, "fk_graph_replay: nothing captured");
   FK_HIP(hipGraphLaunch(g->gexec, S(stream)));
+  g->stat_set = kBatchedSet;
   return ok();
 }
 

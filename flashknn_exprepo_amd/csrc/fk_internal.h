@@ -17,13 +17,20 @@ constexpr int kMaxLaneK = 64;
 enum Scalar : int {
   kStatus = 0,        // first error raised by a kernel (fk_status), 0 = ok
   kTotalCells = 1,    // cells of all per-cloud boxes
-  kPending = 2,       // exact mode: queries left uncertified by pass 1
   kQueueHead = 3,     // batched mode: next work item
-  kStatCertified = 4, // exact mode: certified by pass 1
-  kStatExpanded = 5,  // exact mode: resolved by shell expansion
-  kPending2 = 6,      // batched exact mode: pending list of pass 2 head
-  kNumScalars = 16
+  kCounters = 16,     // query counter sets: kCounters + 3 * set (below)
+  kNumScalars = 32
 };
+
+// One set of query counters.  Plain queries alternate between sets 0 and 1
+// (each first pass clears the other set for the next call, so no separate
+// reset launch is needed); batched launches use set 2, cleared by k_items.
+enum Counter : int {
+  kCntPending = 0,    // exact mode: queries left uncertified by pass 1
+  kCntCertified = 1,  // exact mode: certified by pass 1
+  kCntExpanded = 2    // exact mode: resolved by shell expansion
+};
+constexpr int kBatchedSet = 2;
 
 // Dense cell box of one cloud.  Cell (lx,ly,lz) of cloud c (local
 // coordinates, 0 <= l < n) has linear id base + (lx*ny + ly)*nz + lz, so the
@@ -135,6 +142,8 @@ struct QueryArgs {
   int32_t exact;
   int32_t passes;        // bit 0 reset, bit 1 first pass, bit 2 expansion
   int32_t *scalars;
+  int32_t *qcnt;         // this call's counter set (Counter slots)
+  int32_t *qcnt_next;    // set the first pass clears for the next call (or null)
   int32_t *pend_q;       // [nq]
   uint64_t *pend_keys;   // [nq * k]
   // batched mode
@@ -163,9 +172,12 @@ hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *per_cloud,
                              hipStream_t s);
 hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
                                    hipStream_t s);
+// Per-cloud work-item prefix of a batched launch whose largest k is max_k
+// (one item = batched_tile(max_k) queries).
+int batched_tile(int32_t max_k);
 hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
-                        int32_t B, int32_t *prefix, int32_t *scalars,
-                        hipStream_t s);
+                        int32_t B, int32_t max_k, int32_t *prefix,
+                        int32_t *scalars, int32_t *qcnt, hipStream_t s);
 int64_t scan_parts_needed(int64_t n);
 hipError_t launch_scan_host(const int32_t *in, int32_t *out, int64_t n,
                             int32_t *part, hipStream_t s);

@@ -601,8 +601,8 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
     // one count and one pending slot per query: its group's lane 0
     const bool lead = sub == 0;
     const int nc = __popcll(__ballot(active && done && lead));
-    if (lane == 0 && nc) atomicAdd(a.scalars + kStatCertified, nc);
-    int slot = wave_prefix_slot(active && !done && lead, a.scalars + kPending);
+    if (lane == 0 && nc) atomicAdd(a.qcnt + kCntCertified, nc);
+    int slot = wave_prefix_slot(active && !done && lead, a.qcnt + kCntPending);
     if constexpr (G > 1) slot = __shfl(slot, lane & ~(G - 1), kWave);
     if (active && !done && slot >= a.nq) {
       // the pending counter was not reset before this pass: refuse to write
@@ -636,6 +636,7 @@ k_query(QueryArgs a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   constexpr int G = lanes_per_query<KC>();
+  if (blockIdx.x == 0 && threadIdx.x < 3 && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int q0 = (blk * WPB + w) * (kWave / G);
   // wave-uniform exits: past the end, or the build/attach raised an error
   // (its tables may then be incomplete)
@@ -788,15 +789,21 @@ __device__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &
     }
   };
   advance();
+  // four records of the lane's current column in flight per step
   while (wave_any(p < e)) {
-    const bool has = p < e;
-    uint64_t key = kMaxKey;
-    if (has) {
-      const float4 r = a.srec[p];
-      key = make_key(sqdist(r.x, r.y, r.z, qr.x, qr.y, qr.z), __float_as_uint(r.w));
-      ++p;
+    float4 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) r[u] = a.srec[p + u < e ? p + u : 0];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool has = p + u < e;
+      const uint64_t key =
+          has ? make_key(sqdist(r[u].x, r[u].y, r[u].z, qr.x, qr.y, qr.z),
+                         __float_as_uint(r[u].w))
+              : kMaxKey;
+      t.offer(key, has, lane);
     }
-    t.offer(key, has, lane);
+    p = p + 4 < e ? p + 4 : e;
     if (p >= e) advance();
   }
   if (t.cnt > 0) t.flush(lane);
@@ -830,7 +837,7 @@ __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
 __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.scalars[kPending], a.nq);
+  const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq);
   const int nwaves = gridDim.x * kCoopWaves;
   int nexp = 0;
   for (int slot = blockIdx.x * kCoopWaves + w; slot < npend; slot += nwaves) {
@@ -857,7 +864,7 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
     if (lane < k)
       write_entry(a, out_base(a, ci, qc.w, __float_as_int(qr.w), k) + lane, t.L[0]);
   }
-  if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
+  if (lane == 0 && nexp) atomicAdd(a.qcnt + kCntExpanded, nexp);
 }
 
 // k in (64, 128]: one wave per query from the start, the best list held as
@@ -866,6 +873,7 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
 __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  if (blockIdx.x == 0 && threadIdx.x < 3 && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int nq = a.scalars[kStatus] != 0 ? 0 : a.nq;
   const int nwaves = gridDim.x * kCoopWaves;
   int ncert = 0, nexp = 0;
@@ -892,8 +900,8 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
     if (lane < k) write_entry(a, o + lane, t.L[0]);
     if (lane + kWave < k) write_entry(a, o + kWave + lane, t.L[1]);
   }
-  if (lane == 0 && ncert) atomicAdd(a.scalars + kStatCertified, ncert);
-  if (lane == 0 && nexp) atomicAdd(a.scalars + kStatExpanded, nexp);
+  if (lane == 0 && ncert) atomicAdd(a.qcnt + kCntCertified, ncert);
+  if (lane == 0 && nexp) atomicAdd(a.qcnt + kCntExpanded, nexp);
 }
 
 // ----------------------------------------------------- batched persistent
@@ -920,11 +928,19 @@ constexpr int batched_slots() {
   return m;
 }
 
+// Lanes per query of a batched launch (one value for all its K classes, so
+// that every work item is one tile of kWave / G queries).
+template <int MAXKC>
+constexpr int batched_lanes() {
+  return MAXKC <= 32 ? 2 : 1;
+}
+
 template <int MAXKC>
 __global__ void __launch_bounds__(64 * waves_per_block<MAXKC>())
 k_batched(QueryArgs a) {
   constexpr int WPB = waves_per_block<MAXKC>();
   constexpr int BS = batched_slots<MAXKC>();
+  constexpr int G = batched_lanes<MAXKC>();
   __shared__ uint64_t sbuf[WPB * BS * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   uint64_t *lane_buf = sbuf + w * BS * kWave + lane;
@@ -939,7 +955,7 @@ k_batched(QueryArgs a) {
     const CloudInfo ci = a.clouds[c];
     const int qb = a.cell_start[ci.base];
     const int qe = a.cell_start[ci.base + ci.nx * ci.ny * ci.nz];
-    const int q0 = qb + (item - prefix[c]) * kWave;
+    const int q0 = qb + (item - prefix[c]) * (kWave / G);
     const int k = a.k_per_cloud[c];
     if (k < 1 || k > a.k) {
       // a per-cloud k above the launch's k class: refuse the tile
@@ -947,20 +963,20 @@ k_batched(QueryArgs a) {
       continue;
     }
     if (k <= 8 || MAXKC == 8)
-      run_tile<8, 1>(a, q0, qe, lane_buf);
+      run_tile<8, G>(a, q0, qe, lane_buf);
     else if (k <= 16 || MAXKC == 16)
-      run_tile<(MAXKC < 16 ? MAXKC : 16), 1>(a, q0, qe, lane_buf);
+      run_tile<(MAXKC < 16 ? MAXKC : 16), G>(a, q0, qe, lane_buf);
     else if (k <= 32 || MAXKC == 32)
-      run_tile<(MAXKC < 32 ? MAXKC : 32), 1>(a, q0, qe, lane_buf);
+      run_tile<(MAXKC < 32 ? MAXKC : 32), G>(a, q0, qe, lane_buf);
     else
-      run_tile<MAXKC, 1>(a, q0, qe, lane_buf);
+      run_tile<MAXKC, G>(a, q0, qe, lane_buf);
   }
 }
 
 // Per-cloud tile counts -> exclusive prefix (one block).
 __global__ void __launch_bounds__(1024)
 k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
-        int32_t *prefix, int32_t *scalars) {
+        int32_t tile, int32_t *prefix, int32_t *scalars, int32_t *qcnt) {
   __shared__ int wsum[16];
   __shared__ int carry;
   if (threadIdx.x == 0) carry = 0;
@@ -972,7 +988,7 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
     if (c < B) {
       const CloudInfo ci = clouds[c];
       const int n = cell_start[ci.base + ci.nx * ci.ny * ci.nz] - cell_start[ci.base];
-      tiles = (n + kWave - 1) / kWave;
+      tiles = (n + tile - 1) / tile;
     }
     int v = tiles;
 #pragma unroll
@@ -992,9 +1008,9 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
   if (threadIdx.x == 0) {
     prefix[B] = carry;
     scalars[kQueueHead] = 0;
-    scalars[kPending] = 0;
-    scalars[kStatCertified] = 0;
-    scalars[kStatExpanded] = 0;
+    qcnt[kCntPending] = 0;
+    qcnt[kCntCertified] = 0;
+    qcnt[kCntExpanded] = 0;
   }
 }
 
@@ -1095,13 +1111,8 @@ __global__ void k_zero_u64(unsigned long long *p, int32_t n) {
   if (i < n) p[i] = 0ull;
 }
 
-__global__ void k_reset_query_scalars(int32_t *scalars) {
-  if (threadIdx.x == 0) {
-    scalars[kPending] = 0;
-    scalars[kStatCertified] = 0;
-    scalars[kStatExpanded] = 0;
-    scalars[kQueueHead] = 0;
-  }
+__global__ void k_reset_query_counters(int32_t *qcnt) {
+  if (threadIdx.x < 3) qcnt[threadIdx.x] = 0;
 }
 
 // Fixed grid (graph-capturable): pending queries are read on the device.
@@ -1151,7 +1162,7 @@ int kclass_of(int k) {
 
 hipError_t launch_query(const QueryArgs &a, hipStream_t s) {
   if (a.passes & 1)
-    hipLaunchKernelGGL(k_reset_query_scalars, dim3(1), dim3(64), 0, s, a.scalars);
+    hipLaunchKernelGGL(k_reset_query_counters, dim3(1), dim3(64), 0, s, a.qcnt);
   if (a.nq <= 0) return hipGetLastError();
   if (a.k > kMaxLaneK) {
     if (a.passes & 2) {
@@ -1211,11 +1222,20 @@ hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *acc,
   return hipGetLastError();
 }
 
+int batched_tile(int32_t max_k) {
+  switch (kclass_of(max_k)) {
+    case 8: return kWave / batched_lanes<8>();
+    case 16: return kWave / batched_lanes<16>();
+    case 32: return kWave / batched_lanes<32>();
+    default: return kWave / batched_lanes<64>();
+  }
+}
+
 hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
-                        int32_t B, int32_t *prefix, int32_t *scalars,
-                        hipStream_t s) {
+                        int32_t B, int32_t max_k, int32_t *prefix,
+                        int32_t *scalars, int32_t *qcnt, hipStream_t s) {
   hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, s, clouds, cell_start, B,
-                     prefix, scalars);
+                     batched_tile(max_k), prefix, scalars, qcnt);
   return hipGetLastError();
 }
 

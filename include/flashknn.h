@@ -100,7 +100,7 @@ typedef enum fk_status {
                               fk_grid_status() (graph-capture safe)         */
 
 typedef struct fk_query_params {
-  int32_t k;            /* neighbours per query, 1..64                     */
+  int32_t k;            /* neighbours per query, 1..FK_MAX_K               */
   int32_t mode;         /* FK_MODE_*                                        */
   int32_t radius_cells; /* FK_MODE_CELLS: block half-width (>= 1)          */
   float cut_r2;         /* keep d < cut_r2 (strict); INFINITY = no cut      */
@@ -112,7 +112,11 @@ typedef struct fk_query_params {
   int32_t passes;       /* 0 = all; else a mask of the launches fk_query
                            enqueues: 1 counter reset, 2 first (cell block)
                            pass, 4 expansion pass (FK_MODE_EXACT).  Lets a
-                           profiler time one kernel alone.                  */
+                           profiler time one kernel alone.  With 0 the
+                           counters are reset only when needed (each first
+                           pass clears the next call's set); a caller that
+                           captures fk_query into a graph passes 7 so that
+                           the reset is part of every replay.               */
   int32_t reserved[2];
 } fk_query_params;
 
