@@ -38,6 +38,7 @@ FK_DIST_SQRT = 2
 FK_QUERIES_SELF = 0
 FK_QUERIES_ATTACHED = 1
 FK_BUILD_NO_SYNC = 1
+FK_QUERY_PRUNE = 1
 
 
 class FlashKNNError(RuntimeError):
@@ -60,13 +61,14 @@ class QueryParams(ctypes.Structure):
         ("pad_dist", ctypes.c_float),
         ("query_set", ctypes.c_int32),
         ("passes", ctypes.c_int32),
-        ("reserved", ctypes.c_int32 * 2),
+        ("flags", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
     ]
 
 
 def make_params(k, mode=FK_MODE_CELLS, radius_cells=1, cut_r2=math.inf, idx_64=True,
                 pad_mode=FK_PAD_SELF, dist_mode=FK_DIST_SQ, pad_dist=math.inf,
-                query_set=FK_QUERIES_SELF, passes=0) -> QueryParams:
+                query_set=FK_QUERIES_SELF, passes=0, flags=0) -> QueryParams:
     p = QueryParams()
     p.k = int(k)
     p.mode = int(mode)
@@ -78,6 +80,7 @@ def make_params(k, mode=FK_MODE_CELLS, radius_cells=1, cut_r2=math.inf, idx_64=T
     p.pad_dist = float(pad_dist)
     p.query_set = int(query_set)
     p.passes = int(passes)
+    p.flags = int(flags)
     return p
 
 

@@ -46,22 +46,32 @@ def _load():
     return _TABLE
 
 
-def lookup_target_ratio(n: int, k: int, dclass: int = 0) -> float:
-    """Tuned ratio of the bucket; else the nearest tuned size with the same
-    k class and density class; else the default."""
-    from .knn import DEFAULT_TARGET_RATIO
+def _entry(n: int, k: int, dclass: int):
+    """Table entry of the bucket; else the nearest tuned size with the same
+    k class and density class; else None."""
     with _LOCK:
         t = _load()
     e = t.get(bucket(n, k, dclass))
     if e:
-        return float(e["target_ratio"])
+        return e
     suffix = f"_k{_kclass(k)}_d{int(dclass)}"
     ln = max(0, int(math.log2(max(n, 1))))
     near = [(abs(int(key[1:key.index("_")]) - ln), v) for key, v in t.items()
             if key.endswith(suffix)]
-    if near:
-        return float(min(near, key=lambda x: x[0])[1]["target_ratio"])
-    return DEFAULT_TARGET_RATIO
+    return min(near, key=lambda x: x[0])[1] if near else None
+
+
+def lookup_target_ratio(n: int, k: int, dclass: int = 0) -> float:
+    """Tuned target ratio (see _entry); else the default."""
+    from .knn import DEFAULT_TARGET_RATIO
+    e = _entry(n, k, dclass)
+    return float(e["target_ratio"]) if e else DEFAULT_TARGET_RATIO
+
+
+def lookup_prune(n: int, k: int, dclass: int = 0) -> bool:
+    """Tuned FK_QUERY_PRUNE choice (see _entry); else off."""
+    e = _entry(n, k, dclass)
+    return bool(e.get("prune", False)) if e else False
 
 
 def _time_ms(fn, reps=5):
@@ -77,8 +87,10 @@ def _time_ms(fn, reps=5):
 
 
 def tune(xyz, offsets, k, ratios=RATIOS, save=True):
-    """Time the exact query of one cloud at each target ratio; record the
-    fastest under the cloud's (size, k, density class) bucket."""
+    """Time the exact query of one cloud at each target ratio, with and
+    without FK_QUERY_PRUNE (k <= 16); record the fastest under the cloud's
+    (size, k, density class) bucket.  Returns ((ratio, prune), {(ratio,
+    prune): ms}, density class)."""
     from .knn import ExactKNN, density_class
     eng = ExactKNN()
     eng.build(xyz, offsets, k, adapt=False)
@@ -87,13 +99,17 @@ def tune(xyz, offsets, k, ratios=RATIOS, save=True):
     results = {}
     for r in ratios:
         eng.build(xyz, offsets, k, target_ratio=r)
-        results[r] = _time_ms(lambda: eng.grid.query(k, mode=1, pad_mode=1))
+        for pr in ((False, True) if k <= 16 else (False,)):
+            results[(r, pr)] = _time_ms(
+                lambda: eng.grid.query(k, mode=1, pad_mode=1, prune=pr))
     best = min(results, key=results.get)
     if save:
         with _LOCK:
             t = _load()
-            t[bucket(n, k, dclass)] = {"target_ratio": best, "ms": results,
-                                       "density_class": dclass}
+            t[bucket(n, k, dclass)] = {
+                "target_ratio": best[0], "prune": best[1], "density_class": dclass,
+                "ms": {f"{r}{'p' if pr else ''}": round(v, 4)
+                       for (r, pr), v in results.items()}}
             tmp = TABLE_PATH + ".tmp"
             with open(tmp, "w") as f:
                 json.dump(t, f, indent=1, sort_keys=True)
@@ -126,9 +142,10 @@ def main(argv=None):
             o = torch.tensor([len(pts)], dtype=torch.int32, device=dev)
             for k in (int(s) for s in args.ks.split(",")):
                 best, res, dc = tune(x, o, k)
-                print(f"{name} n={len(pts)} k={k} d={dc}: best ratio {best} "
-                      f"({res[best]:.4f} ms; " +
-                      ", ".join(f"{r}:{m:.4f}" for r, m in sorted(res.items())) + ")",
+                print(f"{name} n={len(pts)} k={k} d={dc}: best ratio {best[0]}"
+                      f"{' prune' if best[1] else ''} ({res[best]:.4f} ms; " +
+                      ", ".join(f"{r}{'p' if pr else ''}:{m:.4f}"
+                                for (r, pr), m in sorted(res.items())) + ")",
                       flush=True)
 
 

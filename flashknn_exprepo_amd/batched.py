@@ -57,7 +57,7 @@ class BatchedKNN:
         self.grid.batched_query(self.k, self.out_offsets, self.out_idx, self.out_dist,
                                 max_k=self.max_k,
                                 mode=F.FK_MODE_EXACT if self.exact else F.FK_MODE_CELLS,
-                                idx_dtype=self.out_idx.dtype)
+                                idx_dtype=self.out_idx.dtype, prune=self.grid.prune_hint)
         return self.out_idx, self.out_dist
 
     def capture(self, stream=None):
@@ -69,7 +69,8 @@ class BatchedKNN:
             self.grid.batched_query(self.k, self.out_offsets, self.out_idx,
                                     self.out_dist, max_k=self.max_k,
                                     mode=F.FK_MODE_EXACT if self.exact else F.FK_MODE_CELLS,
-                                    idx_dtype=self.out_idx.dtype, capture=True)
+                                    idx_dtype=self.out_idx.dtype, capture=True,
+                                    prune=self.grid.prune_hint)
         torch.cuda.current_stream().wait_stream(s)
         self._captured = True
 

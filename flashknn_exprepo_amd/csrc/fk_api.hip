@@ -543,6 +543,7 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.out_d = p->dist_mode == FK_DIST_NONE ? nullptr : out_dist;
   a.exact = p->mode == FK_MODE_EXACT;
   a.passes = p->passes ? (p->passes & 7) : 7;
+  a.prune = (p->flags & FK_QUERY_PRUNE) ? 1 : 0;
   a.scalars = g->scalars;
   a.qcnt = g->scalars + kCounters + 3 * g->qset;
   a.qcnt_next = nullptr;

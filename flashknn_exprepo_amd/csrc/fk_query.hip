@@ -14,9 +14,8 @@
 // one query and a wave owns 64 / G consecutive queries of the cell-sorted
 // order, so its lanes walk the same or adjacent column segments and their
 // candidate loads coalesce or broadcast; lane g of a group takes every G-th
-// record of each column.  Each lane keeps its best KC keys sorted in VGPRs as
-// u64 (d bits << 32 | id): for d >= 0 the u64 order is the (d, id) order, so
-// one 64-bit compare decides both the distance and the tie.  Accepted
+// record of each column.  Each lane keeps its best KC keys sorted in VGPRs
+// (keys below: one compare orders distance and tie together).  Accepted
 // candidates are appended to a per-lane LDS buffer (one ds_write, no
 // divergent insertion); when any lane's buffer is nearly full the wave runs
 // one bitonic sort of the buffer and a bitonic merge into the register
@@ -201,6 +200,7 @@ struct TopK {
   int k;
   int base;         // KC - k: first real slot
   uint64_t cutkey;
+  uint64_t padkey;  // (+inf, pad): an empty slot
   uint64_t thr;     // accept key < thr
 
   __device__ __forceinline__ void init(uint64_t *lane_buf, int k_, float cut_r2,
@@ -210,7 +210,7 @@ struct TopK {
     k = k_;
     base = KC - k_;
     cutkey = make_key(cut_r2, 0u);
-    const uint64_t padkey = key_of_bits(kInfBits, pad);
+    padkey = key_of_bits(kInfBits, pad);
 #pragma unroll
     for (int i = 0; i < KC; ++i) top[i] = i < base ? 0ull : padkey;
     thr = cutkey;  // (inf, pad) >= cutkey for every cut
@@ -289,18 +289,33 @@ struct TopK {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // partner key j (j < k) is read where it is used: no second list
       const uint64_t *pb = lb + ((lane ^ o) - lane);
-      uint64_t p[KC];
 #pragma unroll
-      for (int i = 0; i < KC; ++i) p[i] = i < k ? pb[i * kWave] : kMaxKey;
+      for (int i = 0; i < KC; ++i) {
+        const int j = KC - 1 - i;
+        top[i] = kmin(top[i], j < k ? pb[j * kWave] : kMaxKey);
+      }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-      for (int i = 0; i < KC; ++i) top[i] = kmin(top[i], p[KC - 1 - i]);
       bitonic_merge<KC>(top);
     }
     refresh_thr();
+  }
+
+  // Mid-scan merge: the group's lead lane keeps the merged list, the others
+  // start again from empty slots (so the final merge counts nothing twice),
+  // and every lane takes the merged k-th key as its threshold.
+  template <int G>
+  __device__ __forceinline__ void group_merge_keep(int lane, int sub) {
+    group_merge<G>(lane);
+    if (sub != 0) {
+#pragma unroll
+      for (int i = 0; i < KC; ++i) top[i] = i < base ? 0ull : padkey;
+      thr = cutkey;
+    }
+    share_threshold<G>();
   }
 };
 
@@ -363,6 +378,57 @@ __device__ __forceinline__ void column_range(const CloudInfo &ci,
 // the k-th distance (the acceptance threshold) is tight early and fewer
 // later candidates get buffered and merged.  Then the others in raster
 // order (dx, dy) = (0, 0) is skipped.
+// Exact pruning of neighbour cells against the current threshold (grids
+// built from coordinates, where a cell's extent is known).  In cell units
+// (x * inv_cell), a point of the cell at offset d on an axis lies at least
+// gap(d) from the query on that axis: f or 1 - f plus |d| - 1 whole cells,
+// f the query's position inside its own cell.  The margin m covers the
+// fp32 rounding of x * inv (which decides a point's cell) and of the
+// query's own position; the 1e-5 factors cover the fp32 distance and
+// threshold arithmetic.  A cell is skipped only if every point in it is
+// strictly farther than the threshold, so the result does not change.
+struct Cull {
+  float f[3], m[3];
+  float eff_s;  // threshold distance in cell units^2, inflated; +inf: no pruning
+  __device__ __forceinline__ void setup(const CloudInfo &ci, int4 qc, float4 qr,
+                                        float eff) {
+    const float inv = ci.inv_cell;
+    const float u[3] = {__fmul_rn(qr.x, inv), __fmul_rn(qr.y, inv),
+                        __fmul_rn(qr.z, inv)};
+    const int c[3] = {qc.x + ci.minx, qc.y + ci.miny, qc.z + ci.minz};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      f[i] = u[i] - (float)c[i];
+      m[i] = 4e-7f * (fabsf(u[i]) + 4.f);
+    }
+    eff_s = (inv > 0.f && eff < INFINITY)
+                ? __fmul_rn(__fmul_rn(__fmul_rn(eff, inv), inv), 1.00001f)
+                : INFINITY;
+  }
+  __device__ __forceinline__ float gap2(int ax, int d) const {
+    float g = d < 0 ? f[ax] + (float)(-d - 1) : (d > 0 ? (1.f - f[ax]) + (float)(d - 1) : 0.f);
+    g = g > m[ax] ? g - m[ax] : 0.f;
+    return g * g;
+  }
+  __device__ __forceinline__ bool keep(float lb2) const {
+    return __fmul_rn(lb2, 0.99999f) <= eff_s;
+  }
+};
+
+// Column (qc.x + dx, qc.y + dy), z within qc.z -/+ 1, trimmed to the cells
+// the threshold can still reach: [b, e) (empty when the column is pruned).
+__device__ __forceinline__ void culled_column(const CloudInfo &ci,
+                                              const int32_t *__restrict__ cs,
+                                              const Cull &cu, int4 qc, int dx,
+                                              int dy, int &b, int &e) {
+  b = e = 0;
+  const float xy = cu.gap2(0, dx) + cu.gap2(1, dy);
+  if (!cu.keep(xy)) return;
+  const int z0 = cu.keep(xy + cu.gap2(2, -1)) ? qc.z - 1 : qc.z;
+  const int z1 = cu.keep(xy + cu.gap2(2, 1)) ? qc.z + 1 : qc.z;
+  column_range(ci, cs, qc.x + dx, qc.y + dy, z0, z1, b, e);
+}
+
 // The 8 columns around the query's own (R = 1) as parallel streams: their
 // cell_start reads are issued together, then every round loads the next
 // record of each column (lane `sub` of a group takes every G-th), so the
@@ -370,7 +436,8 @@ __device__ __forceinline__ void column_range(const CloudInfo &ci,
 template <int KC, int G>
 __device__ __forceinline__ void scan_ring8(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
-                                           float4 qr, bool active, int sub) {
+                                           float4 qr, bool active, int sub,
+                                           const Cull &cu) {
   constexpr int kRoom = buffer_slots<KC>() - 8;
   static_assert(kRoom >= 0, "buffer too small for 8 streams");
   int p[8], e[8];
@@ -378,9 +445,7 @@ __device__ __forceinline__ void scan_ring8(TopK<KC> &t, const QueryArgs &a,
   for (int c = 0; c < 8; ++c) {
     const int j = c < 4 ? c : c + 1;  // 3x3 raster order without the centre
     int b = 0, en = 0;
-    if (active)
-      column_range(ci, a.cell_start, qc.x + j / 3 - 1, qc.y + j % 3 - 1, qc.z - 1,
-                   qc.z + 1, b, en);
+    if (active) culled_column(ci, a.cell_start, cu, qc, j / 3 - 1, j % 3 - 1, b, en);
     p[c] = b + sub;
     e[c] = en;
   }
@@ -412,14 +477,28 @@ template <int KC, int G>
 __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
                                            float4 qr, int R, bool active,
-                                           int sub) {
+                                           int sub, int lane) {
+  // K classes 8 and 16, R = 1: the query's own column, then the 8 neighbour
+  // columns as parallel streams.  With FK_QUERY_PRUNE an exact threshold for
+  // the group after the own column (flush + mid-scan merge) prunes the
+  // neighbour columns and their end cells; it pays where the 27-cell block
+  // holds many more candidates than k (large cells, uneven density).  (The
+  // larger classes keep the plain column order: their register lists leave
+  // no room for the extra state.)
   if constexpr (KC <= 16 && buffer_slots<KC>() >= 16) {
     if (R == 1) {
       int b = 0, e = 0;
       if (active)
         column_range(ci, a.cell_start, qc.x, qc.y, qc.z - 1, qc.z + 1, b, e);
       scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
-      scan_ring8<KC, G>(t, a, ci, qc, qr, active, sub);
+      Cull cu;
+      cu.setup(ci, qc, qr, INFINITY);
+      if (a.prune) {  // launch-uniform
+        if (wave_any(t.cnt > 0)) t.flush();
+        if constexpr (G > 1) t.template group_merge_keep<G>(lane, sub);
+        cu.setup(ci, qc, qr, __uint_as_float(key_dbits(t.thr)));
+      }
+      scan_ring8<KC, G>(t, a, ci, qc, qr, active, sub, cu);
       return;
     }
   }
@@ -587,7 +666,7 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
   const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
   TopK<KC> t;
   t.init(lane_buf, k, a.cut_r2, pad_value(a, qr, qs));
-  scan_block<KC, G>(t, a, ci, qc, qr, a.R, active, sub);
+  scan_block<KC, G>(t, a, ci, qc, qr, a.R, active, sub, lane);
   if (wave_any(t.cnt > 0)) t.flush();
   if constexpr (G > 1) t.template group_merge<G>(lane);
 

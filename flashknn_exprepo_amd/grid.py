@@ -22,6 +22,9 @@ def _dev(t: torch.Tensor, dtype: torch.dtype, name: str) -> torch.Tensor:
 class Grid:
     """Cell-sorted support points of one or more clouds on one HIP device."""
 
+    # speed hint for exact queries of the last build_exact_grid (FK_QUERY_PRUNE)
+    prune_hint = False
+
     def __init__(self):
         self._L = F.load()
         h = ctypes.c_void_p()
@@ -184,9 +187,10 @@ class Grid:
     def query(self, k, *, mode=F.FK_MODE_CELLS, radius_cells=1, cut_r2=math.inf,
               idx_dtype=torch.int64, pad_mode=F.FK_PAD_SELF, dist_mode=F.FK_DIST_SQ,
               pad_dist=math.inf, attached=False, out_idx=None, out_dist=None,
-              passes=0):
+              passes=0, prune=False):
         """passes: 0 = full query; else a mask (1 reset, 2 first pass,
-        4 expansion) used to time one kernel alone."""
+        4 expansion) used to time one kernel alone.  prune: FK_QUERY_PRUNE
+        (a speed hint; results are identical)."""
         nq = self.num_queries if attached else self.num_points
         if attached and nq < 0:
             raise RuntimeError("no queries attached")
@@ -198,7 +202,7 @@ class Grid:
         p = F.make_params(k, mode, radius_cells, cut_r2, out_idx.dtype == torch.int64,
                           pad_mode, dist_mode, pad_dist,
                           F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
-                          passes)
+                          passes, F.FK_QUERY_PRUNE if prune else 0)
         F.check(self._L.fk_query(self._h, ctypes.byref(p), F.ptr(out_idx),
                                  F.ptr(out_dist), F.stream_ptr()), "fk_query")
         return out_idx, out_dist
@@ -231,9 +235,11 @@ class Grid:
 
     def batched_query(self, k_per_cloud, out_offsets, out_idx, out_dist, *, max_k,
                       mode=F.FK_MODE_EXACT, radius_cells=1, idx_dtype=torch.int64,
-                      pad_mode=F.FK_PAD_SELF, dist_mode=F.FK_DIST_SQ, capture=False):
+                      pad_mode=F.FK_PAD_SELF, dist_mode=F.FK_DIST_SQ, capture=False,
+                      prune=False):
         p = F.make_params(max_k, mode, radius_cells, math.inf, idx_dtype == torch.int64,
-                          pad_mode, dist_mode, math.inf, F.FK_QUERIES_SELF)
+                          pad_mode, dist_mode, math.inf, F.FK_QUERIES_SELF,
+                          flags=F.FK_QUERY_PRUNE if prune else 0)
         fn = self._L.fk_batched_capture if capture else self._L.fk_batched_query
         F.check(fn(self._h, ctypes.byref(p), F.ptr(k_per_cloud), F.ptr(out_offsets),
                    F.ptr(out_idx), F.ptr(out_dist), F.stream_ptr()),

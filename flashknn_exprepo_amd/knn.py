@@ -91,14 +91,18 @@ def density_class(mean_candidates, k) -> int:
 
 
 def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
-                     target_ratio=None, adapt=True, rounds=None, no_sync=True):
+                     target_ratio=None, adapt=True, rounds=None, no_sync=True,
+                     prune=None):
     """Build ``g`` for exact queries of (xyz, offsets); returns the per-cloud
     cell side.  First guess from the bounding-box density, then (clouds of
     >= ADAPT_MIN_POINTS points) density-feedback rebuilds towards the
     autotuned ``target_ratio * k`` candidates per query.  With ``no_sync``
     the builds use the reserved capacity and read nothing back except the
-    per-cloud first-guess statistic (one read)."""
+    per-cloud first-guess statistic (one read).  Sets ``g.prune_hint``
+    (FK_QUERY_PRUNE for the queries on this build): ``prune`` if given, else
+    the autotuner's choice for the clouds holding most points."""
     n = xyz.shape[0]
+    g.prune_hint = bool(prune) if prune is not None else False
     if cell is not None:
         side = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
         side = side.expand(offsets.shape[0]).contiguous()
@@ -109,15 +113,20 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
     if not adapt or n < ADAPT_MIN_POINTS:
         return side
     if target_ratio is None:
-        from .autotune import lookup_target_ratio
+        from .autotune import lookup_prune, lookup_target_ratio
         ends = offsets.long()
         sizes = torch.diff(ends, prepend=ends.new_zeros(1)).clamp_min(1)
         mean = (g.cloud_candidates().double() / sizes.double()).cpu().tolist()
         ks = k.tolist() if isinstance(k, torch.Tensor) else [int(k)] * len(mean)
         ns = sizes.cpu().tolist()
+        dcs = [density_class(mc, kc) for kc, mc in zip(ks, mean)]
         target_ratio = torch.tensor(
-            [lookup_target_ratio(nc, kc, density_class(mc, kc))
-             for nc, kc, mc in zip(ns, ks, mean)], dtype=torch.float32, device=xyz.device)
+            [lookup_target_ratio(nc, kc, dc) for nc, kc, dc in zip(ns, ks, dcs)],
+            dtype=torch.float32, device=xyz.device)
+        if prune is None:
+            votes = sum(nc if lookup_prune(nc, kc, dc) else -nc
+                        for nc, kc, dc in zip(ns, ks, dcs))
+            g.prune_hint = votes > 0
     for _ in range(ADAPT_ROUNDS if rounds is None else rounds):
         g.adapt_cell(k, side, inv, target_ratio, CANDIDATE_EXPONENT)
         g.build_xyz(xyz, offsets, inv, no_sync=no_sync)
@@ -196,7 +205,8 @@ class ExactKNN:
             g.attach_queries(new_xyz.float().contiguous(),
                              qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32),
                              no_sync=no_sync)
-        out = g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, attached=attached, **qkw)
+        out = g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, attached=attached,
+                      prune=g.prune_hint, **qkw)
         g.status()  # the one host read: reports any device-side error
         return out
 

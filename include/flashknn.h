@@ -117,8 +117,16 @@ typedef struct fk_query_params {
                            pass clears the next call's set); a caller that
                            captures fk_query into a graph passes 7 so that
                            the reset is part of every replay.               */
-  int32_t reserved[2];
+  int32_t flags;        /* FK_QUERY_* hints (results never change)          */
+  int32_t reserved;
 } fk_query_params;
+
+/* fk_query_params.flags */
+#define FK_QUERY_PRUNE 1 /* after the query's own column, skip neighbour
+                            cells whose nearest possible point is beyond the
+                            current k-th distance (grids built from xyz, k
+                            <= 16); faster when the 27-cell block holds many
+                            more candidates than k                         */
 
 typedef struct fk_grid fk_grid;
 

@@ -277,6 +277,47 @@ def test_batched_k_range_errors():
         bk.grid.status()
 
 
+@pytest.mark.parametrize("k", [1, 8, 16])
+def test_exact_prune_changes_nothing(oracle, k):
+    """FK_QUERY_PRUNE (skip neighbour cells beyond the current k-th distance)
+    is a speed hint: exact results equal brute force with and without it,
+    at cell sizes from much smaller to much larger than the k-th distance."""
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd import synthetic
+    rng = np.random.default_rng(22)
+    a = synthetic.subsample_to(synthetic.room(seed=48, target=30000), 9000, rng)
+    b = synthetic.subsample_to(synthetic.lidar(seed=49), 11000, rng)
+    dup = np.repeat(a[:40], 3, 0)                       # exact duplicates: ties at d = 0
+    xyz, off = synthetic.pack([np.concatenate([a, dup]), b])
+    ref_i, ref_d = oracle.brute_knn(xyz, off, xyz, off, k)
+    g = fk.Grid()
+    for cell in (0.04, 0.2, 0.7):
+        inv = torch.tensor([1 / cell, 1 / (2 * cell)], dtype=torch.float32, device=_dev())
+        g.build_xyz(_t(xyz), _t(off), inv)
+        for prune in (False, True):
+            idx, d = g.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, prune=prune)
+            _same(idx, d, ref_i, ref_d)
+
+
+def test_cells_prune_matches_oracle(oracle):
+    """In the 27-cell mode on an xyz-built grid, pruning returns the same
+    rows as the unpruned scan and the oracle's candidate-set kNN."""
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd import synthetic
+    xyz = synthetic.room(seed=48, target=20000)
+    g = fk.Grid()
+    side = np.float32(0.15)
+    inv = torch.tensor([1 / side], dtype=torch.float32, device=_dev())
+    g.build_xyz(_t(xyz), _t(np.array([len(xyz)], np.int32)), inv)
+    i0, d0 = g.query(16, mode=F.FK_MODE_CELLS, prune=False)
+    i1, d1 = g.query(16, mode=F.FK_MODE_CELLS, prune=True)
+    assert torch.equal(i0, i1) and torch.equal(d0, d1)
+    invf = np.float32(inv.item())
+    cells = np.floor((xyz * invf).astype(np.float32)).astype(np.int64)
+    ref_i, ref_d = oracle.grid_knn(xyz, cells, None, xyz, cells, None, np.arange(len(xyz)), 16)
+    _same(i1, d1, ref_i, ref_d)
+
+
 def test_exact_cut_radius(oracle):
     rng = np.random.default_rng(12)
     s = rng.random((5000, 3), dtype=np.float32)

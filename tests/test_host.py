@@ -98,6 +98,12 @@ def test_autotune_lookup_and_density_class(tmp_path, monkeypatch):
     assert autotune.lookup_target_ratio(100000, 16, 3) == 13.0
     assert autotune.lookup_target_ratio(12000, 16, 3) == 6.5      # nearest size
     assert autotune.lookup_target_ratio(100000, 32, 3) == DEFAULT_TARGET_RATIO
+    assert autotune.lookup_prune(100000, 16, 3) is False             # absent: off
+    table[autotune.bucket(100000, 16, 5)] = {"target_ratio": 19.0, "prune": True}
+    p.write_text(json.dumps(table))
+    monkeypatch.setattr(autotune, "_TABLE", None)
+    assert autotune.lookup_prune(90000, 16, 5) is True
+    assert autotune.lookup_target_ratio(90000, 16, 5) == 19.0
 
 
 def test_quotient32_correctly_rounded():
