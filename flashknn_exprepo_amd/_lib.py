@@ -115,6 +115,11 @@ SIGNATURES = [
     ("fk_batched_query", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp, _vp, _vp]),
     ("fk_batched_capture", ctypes.c_int, [_vp, ctypes.POINTER(QueryParams), _vp, _vp, _vp, _vp, _vp]),
     ("fk_graph_replay", ctypes.c_int, [_vp, _vp]),
+    ("fk_idw_interp_workspace_bytes", ctypes.c_int64, [_i64, _i64, _i32]),
+    ("fk_idw_interp_forward", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _i32,
+                                             ctypes.c_float, _vp, _vp, _vp]),
+    ("fk_idw_interp_backward", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _i32,
+                                              ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("fk_voxel_workspace_bytes", ctypes.c_int64, [_i64]),
     ("fk_voxel_first", ctypes.c_int, [_vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp, _vp, _vp]),
     ("fk_backward_workspace_bytes", ctypes.c_int64, [_i64, _i64, _i32]),

@@ -7,7 +7,10 @@
 * ``IDWGather``: out_i = sum_j w_ij feat[idx_ij] (fk_idw_forward) with
   backward grad_feat (segmented, deterministic) and grad_w (per entry).
 * ``idw_interpolate``: weights w = 1/(d + eps) on the squared distance,
-  normalised per row.  Unlike pointops' 1/(sqrt(d) + 1e-8)
+  normalised per row, as one fused forward (fk_idw_interp_forward:
+  distances, weights and gather in one wave per row) and one fused backward
+  (fk_idw_interp_backward: gradients for features, queries and supports
+  over a single reverse CSR).  Unlike pointops' 1/(sqrt(d) + 1e-8)
   (Pointcept/libs/pointops/functions/interpolation.py:15-17), whose gradient
   is unbounded at d = 0 (coarse points are a subset of the fine ones, so
   self matches have d = 0), this form is smooth there.
@@ -118,12 +121,53 @@ class IDWGather(Function):
 idw_gather = IDWGather.apply
 
 
+class IDWInterp(Function):
+    """out_i = sum_j w_ij feat[idx_ij], w_ij = r_ij / sum_l r_il,
+    r_ij = 1 / (||p_idx - q_i||^2 + eps) (0 for idx < 0)."""
+
+    @staticmethod
+    def forward(ctx, feat, q, p, idx, eps):
+        feat = feat.float().contiguous()
+        q = q.float().contiguous()
+        p = p.float().contiguous()
+        idx = idx.long().contiguous()
+        nq, k = idx.shape
+        if k > 64:
+            raise ValueError("idw_interpolate supports k <= 64")
+        out = torch.empty(nq, feat.shape[1], dtype=torch.float32, device=feat.device)
+        L = F.load()
+        F.check(L.fk_idw_interp_forward(F.ptr(q), nq, F.ptr(p), p.shape[0], F.ptr(idx), k,
+                                        F.ptr(feat), feat.shape[1], float(eps), None,
+                                        F.ptr(out), F.stream_ptr()), "fk_idw_interp_forward")
+        ctx.save_for_backward(feat, q, p, idx)
+        ctx.eps = float(eps)
+        ctx.mark_non_differentiable(idx)
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        feat, q, p, idx = ctx.saved_tensors
+        nq, k = idx.shape
+        ns, C = feat.shape
+        need_f, need_q, need_p = ctx.needs_input_grad[:3]
+        gf = torch.empty_like(feat) if need_f else None
+        gq = torch.empty_like(q) if need_q else None
+        gp = torch.empty_like(p) if need_p else None
+        L = F.load()
+        nbytes = int(L.fk_idw_interp_workspace_bytes(nq, ns, k))
+        ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=feat.device)
+        g = grad_out.float().contiguous()
+        F.check(L.fk_idw_interp_backward(F.ptr(q), nq, F.ptr(p), ns, F.ptr(idx), k,
+                                         F.ptr(feat), C, ctx.eps, F.ptr(g), F.ptr(gf),
+                                         F.ptr(gq), F.ptr(gp), F.ptr(ws), F.stream_ptr()),
+                "fk_idw_interp_backward")
+        return gf, gq, gp, None, None
+
+
 def idw_interpolate(feat, q, p, idx, eps=1e-8, d=None):
     """Interpolate feat (on support p) at queries q from their neighbours idx.
 
-    Differentiable w.r.t. feat, q and p (idx constant)."""
-    d2 = gathered_sqdist(q, p, idx, d)
-    w = 1.0 / (d2 + eps)
-    w = torch.where(idx >= 0, w, torch.zeros_like(w))
-    wn = w / w.sum(1, keepdim=True)
-    return idw_gather(feat, wn, idx)
+    Differentiable w.r.t. feat, q and p (idx constant).  The distances are
+    recomputed in the query kernels' fp32 form, so a ``d`` returned by the
+    kNN query equals them and is not needed (accepted for compatibility)."""
+    return IDWInterp.apply(feat, q, p, idx, eps)

@@ -782,6 +782,38 @@ int fk_graph_replay(fk_grid *g, void *stream) {
   return ok();
 }
 
+int64_t fk_idw_interp_workspace_bytes(int64_t nq, int64_t ns, int32_t k) {
+  if (nq < 0 || ns < 0 || k < 1 || k > 64) return -1;
+  return interp_workspace_bytes(nq, ns, k);
+}
+
+int fk_idw_interp_forward(const float *q, int64_t nq, const float *p, int64_t ns,
+                          const int64_t *idx, int32_t k, const float *feat,
+                          int32_t channels, float eps, float *w, float *out,
+                          void *stream) {
+  if (nq < 0 || ns < 0 || k < 1 || k > 64 || channels < 1 || nq * (int64_t)k > INT32_MAX ||
+      ns > INT32_MAX || (nq > 0 && (!q || !idx || !out)) || (ns > 0 && (!p || !feat)) ||
+      !std::isfinite(eps) || eps < 0.f)
+    return fail(FK_ERR_INVALID_ARG, "fk_idw_interp_forward: bad arguments");
+  FK_HIP(launch_interp_forward(q, nq, p, ns, idx, k, feat, channels, eps, w, out,
+                               S(stream)));
+  return ok();
+}
+
+int fk_idw_interp_backward(const float *q, int64_t nq, const float *p, int64_t ns,
+                           const int64_t *idx, int32_t k, const float *feat,
+                           int32_t channels, float eps, const float *grad_out,
+                           float *grad_feat, float *grad_q, float *grad_p,
+                           void *workspace, void *stream) {
+  if (nq < 0 || ns < 0 || k < 1 || k > 64 || channels < 1 || nq * (int64_t)k > INT32_MAX ||
+      ns > INT32_MAX || (nq > 0 && (!q || !idx || !grad_out)) ||
+      (ns > 0 && (!p || !feat)) || !workspace || !std::isfinite(eps) || eps < 0.f)
+    return fail(FK_ERR_INVALID_ARG, "fk_idw_interp_backward: bad arguments");
+  FK_HIP(launch_interp_backward(q, nq, p, ns, idx, k, feat, channels, eps, grad_out,
+                                grad_feat, grad_q, grad_p, workspace, S(stream)));
+  return ok();
+}
+
 int64_t fk_voxel_workspace_bytes(int64_t n) {
   if (n < 0 || n > INT32_MAX / 2) return -1;
   return voxel_workspace_bytes(n);

@@ -243,6 +243,124 @@ __global__ void k_idw_bw_w(const float *feat, int64_t ns, int32_t C,
   if (ok && sub == 0) gw[e] = acc;
 }
 
+// ---------------------------------------------------- fused IDW interpolation
+//
+// out_i = sum_j w_ij feat[m_ij], w_ij = r_ij / S_i, r_ij = 1 / (d_ij + eps),
+// S_i = sum_j r_ij, d_ij = ||p_m - q_i||^2 in the query kernels' fp32 form
+// (r = 0 for padded slots).  One wave per query row: lane j owns neighbour j
+// (k <= 64), lane c owns channel c of a 64-channel chunk; every sum over j
+// or over channels runs in a fixed order (ascending j, or a fixed
+// butterfly), so results are reproducible.
+constexpr int kRowWaves = 4;
+
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Lane j < k: its neighbour m (-1 if padded or out of range) and r_j.
+__device__ __forceinline__ void interp_lane(const float *q, int64_t i,
+                                            const float *p, int64_t ns,
+                                            const int64_t *idx, int32_t k,
+                                            float eps, int lane, int &m, float &r) {
+  m = -1;
+  r = 0.f;
+  if (lane < k) {
+    const int64_t mm = idx[i * k + lane];
+    if (mm >= 0 && mm < ns) {
+      m = (int)mm;
+      const float dx = __fsub_rn(p[3 * mm], q[3 * i]);
+      const float dy = __fsub_rn(p[3 * mm + 1], q[3 * i + 1]);
+      const float dz = __fsub_rn(p[3 * mm + 2], q[3 * i + 2]);
+      const float d = __fadd_rn(__fadd_rn(__fmul_rn(dx, dx), __fmul_rn(dy, dy)),
+                                __fmul_rn(dz, dz));
+      r = __fdiv_rn(1.f, __fadd_rn(d, eps));
+    }
+  }
+}
+
+__global__ void __launch_bounds__(64 * kRowWaves)
+k_interp_fwd(const float *q, int64_t nq, const float *p, int64_t ns,
+             const int64_t *idx, int32_t k, const float *feat, int32_t C,
+             float eps, float *w_out, float *out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * kRowWaves + (threadIdx.x >> 6);
+  if (i >= nq) return;  // wave-uniform
+  int m;
+  float r;
+  interp_lane(q, i, p, ns, idx, k, eps, lane, m, r);
+  const float S = wave_sum(r);
+  const float w = S > 0.f ? __fdiv_rn(r, S) : 0.f;
+  if (w_out && lane < k) w_out[i * k + lane] = w;
+  for (int c0 = 0; c0 < C; c0 += 64) {
+    const int c = c0 + lane;
+    float acc = 0.f;
+    for (int j = 0; j < k; ++j) {
+      const int mj = __shfl(m, j, 64);
+      const float wj = __shfl(w, j, 64);
+      if (mj >= 0 && c < C)
+        acc = __fadd_rn(acc, __fmul_rn(wj, feat[(int64_t)mj * C + c]));
+    }
+    if (c < C) out[i * C + c] = acc;
+  }
+}
+
+// Per row: a_j = <g_i, feat[m_j]>, b = sum_j w_j a_j, dL/dd_j =
+// -r_j^2 (a_j - b) / S, stored to gd (and w to w_out) for the support-side
+// sums; dL/dq_i = sum_j 2 dL/dd_j (q_i - p_j).
+__global__ void __launch_bounds__(64 * kRowWaves)
+k_interp_bw_row(const float *q, int64_t nq, const float *p, int64_t ns,
+                const int64_t *idx, int32_t k, const float *feat, int32_t C,
+                float eps, const float *gout, float *w_out, float *gd,
+                float *gq) {
+  const int lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * kRowWaves + (threadIdx.x >> 6);
+  if (i >= nq) return;  // wave-uniform
+  int m;
+  float r;
+  interp_lane(q, i, p, ns, idx, k, eps, lane, m, r);
+  const float S = wave_sum(r);
+  const float w = S > 0.f ? __fdiv_rn(r, S) : 0.f;
+  float a = 0.f;
+  for (int j = 0; j < k; ++j) {
+    const int mj = __shfl(m, j, 64);
+    float part = 0.f;
+    if (mj >= 0)
+      for (int c0 = 0; c0 < C; c0 += 64) {
+        const int c = c0 + lane;
+        if (c < C)
+          part = __fadd_rn(part, __fmul_rn(gout[i * C + c], feat[(int64_t)mj * C + c]));
+      }
+    part = wave_sum(part);
+    if (lane == j) a = part;
+  }
+  const float b = wave_sum(lane < k ? __fmul_rn(w, a) : 0.f);
+  float g = 0.f;
+  if (m >= 0 && S > 0.f) g = -__fmul_rn(__fmul_rn(r, r), __fdiv_rn(__fsub_rn(a, b), S));
+  if (lane < k) {
+    if (w_out) w_out[i * k + lane] = w;
+    if (gd) gd[i * k + lane] = g;
+  }
+  if (gq) {
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (m >= 0) {
+      const float c2 = __fmul_rn(2.f, g);
+      gx = __fmul_rn(c2, __fsub_rn(q[3 * i], p[3 * (int64_t)m]));
+      gy = __fmul_rn(c2, __fsub_rn(q[3 * i + 1], p[3 * (int64_t)m + 1]));
+      gz = __fmul_rn(c2, __fsub_rn(q[3 * i + 2], p[3 * (int64_t)m + 2]));
+    }
+    gx = wave_sum(gx);
+    gy = wave_sum(gy);
+    gz = wave_sum(gz);
+    if (lane == 0) {
+      gq[3 * i] = gx;
+      gq[3 * i + 1] = gy;
+      gq[3 * i + 2] = gz;
+    }
+  }
+}
+
 // Reverse CSR of idx: segment m = [start[m], start[m+1]) of *sorted holds the
 // entries e with idx[e] == m, ascending.
 hipError_t build_reverse_csr(const int64_t *idx, int64_t E, int64_t ns,
@@ -322,6 +440,53 @@ hipError_t launch_idw_backward(const float *feat, int64_t ns, int32_t C,
     if (err != hipSuccess) return err;
     hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * C)), dim3(kBlock), 0, s,
                        ns, C, w, k, grad_out, wsp.start, sorted, grad_feat);
+  }
+  return hipGetLastError();
+}
+
+int64_t interp_workspace_bytes(int64_t nq, int64_t ns, int32_t k) {
+  const int64_t E = nq * k;
+  return 2 * align_up(4 * E) + backward_workspace_bytes(nq, ns, k);
+}
+
+hipError_t launch_interp_forward(const float *q, int64_t nq, const float *p,
+                                 int64_t ns, const int64_t *idx, int32_t k,
+                                 const float *feat, int32_t C, float eps,
+                                 float *w_out, float *out, hipStream_t s) {
+  if (nq <= 0) return hipSuccess;
+  const int64_t nb = (nq + kRowWaves - 1) / kRowWaves;
+  hipLaunchKernelGGL(k_interp_fwd, dim3((unsigned)nb), dim3(64 * kRowWaves), 0, s,
+                     q, nq, p, ns, idx, k, feat, C, eps, w_out, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_interp_backward(const float *q, int64_t nq, const float *p,
+                                  int64_t ns, const int64_t *idx, int32_t k,
+                                  const float *feat, int32_t C, float eps,
+                                  const float *grad_out, float *grad_feat,
+                                  float *grad_q, float *grad_p, void *ws,
+                                  hipStream_t s) {
+  const int64_t E = nq * k;
+  char *base = (char *)ws;
+  float *w = (float *)base;
+  float *gd = (float *)(base + align_up(4 * E));
+  void *csr_ws = base + 2 * align_up(4 * E);
+  if (nq > 0) {
+    const int64_t nb = (nq + kRowWaves - 1) / kRowWaves;
+    hipLaunchKernelGGL(k_interp_bw_row, dim3((unsigned)nb), dim3(64 * kRowWaves), 0, s,
+                       q, nq, p, ns, idx, k, feat, C, eps, grad_out, w, gd, grad_q);
+  }
+  if ((grad_p || grad_feat) && ns > 0) {
+    const Workspace wsp = carve(csr_ws, ns, E);
+    const int32_t *sorted = nullptr;
+    hipError_t err = build_reverse_csr(idx, E, ns, wsp, &sorted, s);
+    if (err != hipSuccess) return err;
+    if (grad_p)
+      hipLaunchKernelGGL(k_dist_bw_p, dim3(nblocks(ns)), dim3(kBlock), 0, s, q, p, ns,
+                         k, gd, wsp.start, sorted, grad_p);
+    if (grad_feat && ns * C > 0)
+      hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * C)), dim3(kBlock), 0, s,
+                         ns, C, w, k, grad_out, wsp.start, sorted, grad_feat);
   }
   return hipGetLastError();
 }

@@ -204,5 +204,18 @@ hipError_t launch_idw_backward(const float *feat, int64_t ns, int32_t C,
                                int32_t k, const float *grad_out,
                                float *grad_feat, float *grad_w, void *ws,
                                hipStream_t s);
+// fused IDW interpolation (distances, normalised weights, gather; backward
+// to features, queries and supports over one reverse CSR)
+int64_t interp_workspace_bytes(int64_t nq, int64_t ns, int32_t k);
+hipError_t launch_interp_forward(const float *q, int64_t nq, const float *p,
+                                 int64_t ns, const int64_t *idx, int32_t k,
+                                 const float *feat, int32_t C, float eps,
+                                 float *w_out, float *out, hipStream_t s);
+hipError_t launch_interp_backward(const float *q, int64_t nq, const float *p,
+                                  int64_t ns, const int64_t *idx, int32_t k,
+                                  const float *feat, int32_t C, float eps,
+                                  const float *grad_out, float *grad_feat,
+                                  float *grad_q, float *grad_p, void *ws,
+                                  hipStream_t s);
 
 }  // namespace fk

@@ -284,6 +284,27 @@ int fk_idw_backward(const float *feat, int64_t ns, int32_t channels,
                     const float *grad_out, float *grad_feat, float *grad_w,
                     void *workspace, void *stream);
 
+/* Fused IDW interpolation over neighbour lists (no counterpart in the
+ * reference, whose pointops interpolation differentiates features only,
+ * Pointcept/libs/pointops/functions/interpolation.py:8-56):
+ *   d_ij = ||p_idx(i,j) - q_i||^2 (the query kernels' fp32 form),
+ *   r_ij = 1 / (d_ij + eps) (0 for idx < 0), w_ij = r_ij / sum_l r_il,
+ *   out[i,:] = sum_j w_ij feat[idx_ij,:]    (out float[nq,C]).
+ * k <= 64.  w (float[nq,k]) may be NULL.  The backward writes any non-NULL
+ * of grad_feat [ns,C], grad_q [nq,3], grad_p [ns,3] (overwritten; the
+ * support-side sums run over a reverse CSR in entry order: reproducible);
+ * workspace: fk_idw_interp_workspace_bytes(nq, ns, k) bytes (device). */
+int64_t fk_idw_interp_workspace_bytes(int64_t nq, int64_t ns, int32_t k);
+int fk_idw_interp_forward(const float *q, int64_t nq, const float *p, int64_t ns,
+                          const int64_t *idx, int32_t k, const float *feat,
+                          int32_t channels, float eps, float *w, float *out,
+                          void *stream);
+int fk_idw_interp_backward(const float *q, int64_t nq, const float *p, int64_t ns,
+                           const int64_t *idx, int32_t k, const float *feat,
+                           int32_t channels, float eps, const float *grad_out,
+                           float *grad_feat, float *grad_q, float *grad_p,
+                           void *workspace, void *stream);
+
 /* -------------------------------------------- voxel-first subsampling */
 /* Replaces _voxel_first_cuda (Networks/hierarchy.py:23-36), the step that
  * builds every coarser level of the hierarchy before the kNN path.  Voxel

@@ -433,6 +433,42 @@ def test_backward_deterministic():
             assert torch.equal(a, b)
 
 
+def test_idw_interp_padding_and_channels(oracle):
+    """Fused IDW forward/backward with padded slots (idx = -1), a fully
+    padded row and 70 channels (two 64-lane chunks) against an FP64 torch
+    reference on the CPU."""
+    rng = np.random.default_rng(23)
+    p = rng.random((500, 3), dtype=np.float32)
+    q = rng.random((300, 3), dtype=np.float32)
+    idx, _ = oracle.brute_knn(p, [500], q, [300], 8)
+    idx[5, 3:] = -1
+    idx[7, :] = -1
+    feat = rng.standard_normal((500, 70)).astype(np.float32)
+    wout = rng.standard_normal((300, 70)).astype(np.float32)
+    eps = 1e-6
+    qt, pt, ft = (_t(a).requires_grad_(True) for a in (q, p, feat))
+    out = fk.idw_interpolate(ft, qt, pt, _t(idx), eps=eps)
+    (out * _t(wout)).sum().backward()
+    it = torch.from_numpy(idx)
+    q_, p_, f_ = (torch.from_numpy(a.astype(np.float64)).requires_grad_(True)
+                  for a in (q, p, feat))
+    ok = it >= 0
+    safe = it.clamp_min(0)
+    d_ = ((q_[:, None, :] - p_[safe]) ** 2).sum(-1)
+    w_ = torch.where(ok, 1.0 / (d_ + eps), torch.zeros_like(d_))
+    S = w_.sum(1, keepdim=True)
+    wn_ = torch.where(S > 0, w_ / S.clamp_min(1e-300), torch.zeros_like(w_))
+    out_ = (wn_[..., None] * f_[safe]).sum(1)
+    (out_ * torch.from_numpy(wout.astype(np.float64))).sum().backward()
+    got = out.detach().double().cpu()
+    assert torch.all(got[7] == 0)
+    assert (torch.linalg.norm(got - out_.detach()) / torch.linalg.norm(out_.detach())) <= 1e-5
+    for g, r in ((qt.grad, q_.grad), (pt.grad, p_.grad), (ft.grad, f_.grad)):
+        g = g.double().cpu()
+        assert (torch.linalg.norm(g - r) / torch.linalg.norm(r)) <= 1e-5
+    assert torch.all(qt.grad[7] == 0)
+
+
 def test_idw_fp64_finite_differences(oracle):
     """C5 check on a 2k-point copy: gradients of the fp32 path vs FP64
     central differences of the same loss, relative error <= 1e-5."""
