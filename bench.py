@@ -102,7 +102,7 @@ def main():
 
     def step():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
-                out_dist=out_d)
+                out_dist=out_d, prune=g.prune_hint)
 
     log(f"headline: n={n} K={K} warmup={args.warmup} steps={args.steps}")
     for _ in range(args.warmup):
@@ -126,7 +126,7 @@ def main():
     # dominant kernel alone (first pass), HIP events on the current stream
     def pass1():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
-                out_dist=out_d, passes=2)
+                out_dist=out_d, passes=2, prune=g.prune_hint)
 
     def reset():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
@@ -166,13 +166,14 @@ def main():
     extra = {"build_ms": round(build_ms, 4), "cells": ncells,
              "certified_first_pass": stats["certified_first_pass"],
              "expanded": stats["expanded"], "bitexact_sample_2000": exact_ok,
-             "cell_side_m": round(float(cell[0]), 5)}
+             "cell_side_m": round(float(cell[0]), 5), "prune": bool(g.prune_hint)}
 
     if not args.quick:
         def qps(grid, k, mode, reps=20, **kw):
             oi = torch.empty((grid.num_points, k), dtype=torch.int64, device=dev)
             od = torch.empty((grid.num_points, k), dtype=torch.float32, device=dev)
-            f = lambda: grid.query(k, mode=mode, out_idx=oi, out_dist=od, **kw)  # noqa: E731
+            f = lambda: grid.query(k, mode=mode, out_idx=oi, out_dist=od,  # noqa: E731
+                                   prune=grid.prune_hint, **kw)
             f()
             med, _ = events_ms(f, reps)
             return grid.num_points / (med * 1e-3) / 1e6, med
@@ -225,7 +226,7 @@ def main():
             e = fk.ExactKNN()
             cx = T(cl)
             co = T(np.array([len(cl)], np.int32))
-            e.build(cx, co, k, cell=bk.cell[c:c + 1])
+            e.build(cx, co, k, cell=bk.cell[c:c + 1], prune=bk.grid.prune_hint)
             oi = torch.empty((len(cl), k), dtype=torch.int64, device=dev)
             od = torch.empty((len(cl), k), dtype=torch.float32, device=dev)
             engines.append((e.grid, k, oi, od, start))
@@ -234,7 +235,7 @@ def main():
         def per_cloud():
             for gr, k, oi, od, _ in engines:
                 gr.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_SELF, out_idx=oi,
-                         out_dist=od)
+                         out_dist=od, prune=gr.prune_hint)
 
         pc_med, _ = events_ms(per_cloud, 20)
         same = True

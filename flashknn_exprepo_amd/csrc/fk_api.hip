@@ -545,7 +545,7 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.passes = p->passes ? (p->passes & 7) : 7;
   a.prune = (p->flags & FK_QUERY_PRUNE) ? 1 : 0;
   a.scalars = g->scalars;
-  a.qcnt = g->scalars + kCounters + 3 * g->qset;
+  a.qcnt = g->scalars + kCounters + kCntStride * g->qset;
   a.qcnt_next = nullptr;
   return FK_OK;
 }
@@ -567,8 +567,8 @@ int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
   const bool first = a.passes & 2;
   const int set = first ? (g->qset ^ 1) : g->qset;
   const bool reset = (p->passes && (a.passes & 1)) || (first && !g->qclean[set]);
-  a.qcnt = g->scalars + kCounters + 3 * set;
-  a.qcnt_next = first && a.nq > 0 ? g->scalars + kCounters + 3 * (set ^ 1) : nullptr;
+  a.qcnt = g->scalars + kCounters + kCntStride * set;
+  a.qcnt_next = first && a.nq > 0 ? g->scalars + kCounters + kCntStride * (set ^ 1) : nullptr;
   a.passes = (a.passes & ~1) | (reset ? 1 : 0);
   FK_HIP(launch_query(a, S(stream)));
   g->qset = set;
@@ -585,7 +585,7 @@ int fk_query_stats(fk_grid *g, int64_t *first_pass_certified, int64_t *expanded,
                    void *stream) {
   FK_CHECK(check_stream_device(g));
   FK_CHECK(read_scalars(g, S(stream)));
-  const int32_t *c = g->h_pinned + kCounters + 3 * g->stat_set;
+  const int32_t *c = g->h_pinned + kCounters + kCntStride * g->stat_set;
   if (first_pass_certified) *first_pass_certified = c[kCntCertified];
   if (expanded) *expanded = c[kCntExpanded];
   return ok();
@@ -721,7 +721,7 @@ static int batched_args(fk_grid *g, const fk_query_params *p,
   }
   a.k_per_cloud = k_per_cloud;
   a.out_offsets = out_offsets;
-  a.qcnt = g->scalars + kCounters + 3 * kBatchedSet;  // cleared by k_items
+  a.qcnt = g->scalars + kCounters + kCntStride * kBatchedSet;  // cleared by k_items
   a.qcnt_next = nullptr;
   a.num_clouds = g->n_clouds;
   a.item_cloud = g->item_prefix;

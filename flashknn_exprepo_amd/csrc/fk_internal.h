@@ -18,7 +18,7 @@ enum Scalar : int {
   kStatus = 0,        // first error raised by a kernel (fk_status), 0 = ok
   kTotalCells = 1,    // cells of all per-cloud boxes
   kQueueHead = 3,     // batched mode: next work item
-  kCounters = 16,     // query counter sets: kCounters + 3 * set (below)
+  kCounters = 16,     // query counter sets: kCounters + kCntStride * set
   kNumScalars = 32
 };
 
@@ -28,7 +28,8 @@ enum Scalar : int {
 enum Counter : int {
   kCntPending = 0,    // exact mode: queries left uncertified by pass 1
   kCntCertified = 1,  // exact mode: certified by pass 1
-  kCntExpanded = 2    // exact mode: resolved by shell expansion
+  kCntExpanded = 2,   // exact mode: resolved by shell expansion
+  kCntStride = 4      // slots per set (one spare)
 };
 constexpr int kBatchedSet = 2;
 

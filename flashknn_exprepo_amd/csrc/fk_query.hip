@@ -715,7 +715,7 @@ k_query(QueryArgs a) {
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   constexpr int G = lanes_per_query<KC>();
-  if (blockIdx.x == 0 && threadIdx.x < 3 && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int q0 = (blk * WPB + w) * (kWave / G);
   // wave-uniform exits: past the end, or the build/attach raised an error
   // (its tables may then be incomplete)
@@ -917,8 +917,10 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq);
-  const int nwaves = gridDim.x * kCoopWaves;
   int nexp = 0;
+  // static striding over the pending list (a shared work counter measured
+  // slower: its atomics cost more than the balance they buy)
+  const int nwaves = gridDim.x * kCoopWaves;
   for (int slot = blockIdx.x * kCoopWaves + w; slot < npend; slot += nwaves) {
     const int qs = a.pend_q[slot];
     const float4 qr = a.qrec[qs];
@@ -952,7 +954,7 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
 __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  if (blockIdx.x == 0 && threadIdx.x < 3 && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int nq = a.scalars[kStatus] != 0 ? 0 : a.nq;
   const int nwaves = gridDim.x * kCoopWaves;
   int ncert = 0, nexp = 0;
@@ -1191,7 +1193,7 @@ __global__ void k_zero_u64(unsigned long long *p, int32_t n) {
 }
 
 __global__ void k_reset_query_counters(int32_t *qcnt) {
-  if (threadIdx.x < 3) qcnt[threadIdx.x] = 0;
+  if (threadIdx.x < kCntStride) qcnt[threadIdx.x] = 0;
 }
 
 // Fixed grid (graph-capturable): pending queries are read on the device.

@@ -152,8 +152,9 @@ class ExactKNN:
         self._cap = (n2, B2, q2)
 
     def build(self, xyz, offsets, k, *, cell=None, scale=None, target_ratio=None,
-              adapt=True, nq=0, no_sync=True):
-        """Build the support grid; returns the per-cloud cell side."""
+              adapt=True, nq=0, no_sync=True, prune=None):
+        """Build the support grid; returns the per-cloud cell side.  The
+        FK_QUERY_PRUNE choice for queries on it is ``self.grid.prune_hint``."""
         F.require_device(xyz, "xyz")
         xyz = xyz.float().contiguous()
         offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
@@ -161,7 +162,7 @@ class ExactKNN:
             self._reserve(xyz.shape[0], offsets.shape[0], max(nq, xyz.shape[0]))
         self.side = build_exact_grid(self.grid, xyz, offsets, k, cell=cell, scale=scale,
                                      target_ratio=target_ratio, adapt=adapt,
-                                     no_sync=no_sync)
+                                     no_sync=no_sync, prune=prune)
         return self.side
 
     @torch.no_grad()
