@@ -102,7 +102,8 @@ def main():
 
     def step():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
-                out_dist=out_d, prune=g.prune_hint)
+                out_dist=out_d, prune=g.prune_hint,
+                one_lane=g.one_lane_hint)
 
     log(f"headline: n={n} K={K} warmup={args.warmup} steps={args.steps}")
     for _ in range(args.warmup):
@@ -126,7 +127,8 @@ def main():
     # dominant kernel alone (first pass), HIP events on the current stream
     def pass1():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
-                out_dist=out_d, passes=2, prune=g.prune_hint)
+                out_dist=out_d, passes=2, prune=g.prune_hint,
+                one_lane=g.one_lane_hint)
 
     def reset():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
@@ -166,14 +168,16 @@ def main():
     extra = {"build_ms": round(build_ms, 4), "cells": ncells,
              "certified_first_pass": stats["certified_first_pass"],
              "expanded": stats["expanded"], "bitexact_sample_2000": exact_ok,
-             "cell_side_m": round(float(cell[0]), 5), "prune": bool(g.prune_hint)}
+             "cell_side_m": round(float(cell[0]), 5), "prune": bool(g.prune_hint),
+             "one_lane": bool(g.one_lane_hint)}
 
     if not args.quick:
         def qps(grid, k, mode, reps=20, **kw):
             oi = torch.empty((grid.num_points, k), dtype=torch.int64, device=dev)
             od = torch.empty((grid.num_points, k), dtype=torch.float32, device=dev)
             f = lambda: grid.query(k, mode=mode, out_idx=oi, out_dist=od,  # noqa: E731
-                                   prune=grid.prune_hint, **kw)
+                                   prune=grid.prune_hint,
+                                   one_lane=grid.one_lane_hint, **kw)
             f()
             med, _ = events_ms(f, reps)
             return grid.num_points / (med * 1e-3) / 1e6, med
@@ -202,6 +206,20 @@ def main():
         v, ms = qps(el.grid, K, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
         extra["lidar_n"] = len(lid)
         extra["lidar_k16_Mqps"], extra["lidar_k16_ms"] = round(v, 2), round(ms, 4)
+        extra["lidar_prune"] = bool(el.grid.prune_hint)
+        # the LiDAR config's own semantics: the adaptive neighbourhood
+        # (per-point octree level by the 2K/8K candidate rule) on 5 cm voxel
+        # coordinates; whole call timed (level builds + selection + query),
+        # set recall against the exact kNN
+        exact_idx, _ = el(lx, lo, K)
+        gl = torch.floor(lx / 0.05).long()
+        lb = torch.zeros(len(lid), dtype=torch.long, device=dev)
+        an = fk.AdaptiveNeighborhoodFlashKNN(num_nbr=K)
+        ad_idx = an.query(gl, lb, lx)
+        ad_med, _ = events_ms(lambda: an.query(gl, lb, lx), 10)
+        extra["lidar_adaptive_k16_Mqps"] = round(len(lid) / (ad_med * 1e-3) / 1e6, 2)
+        extra["lidar_adaptive_k16_ms"] = round(ad_med, 4)
+        extra["lidar_adaptive_recall"] = round(float(fk.cal_recall(ad_idx, exact_idx).mean()), 5)
 
         # uniform 8192
         uni = synthetic.uniform(8192, 47)
@@ -235,7 +253,7 @@ def main():
         def per_cloud():
             for gr, k, oi, od, _ in engines:
                 gr.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_SELF, out_idx=oi,
-                         out_dist=od, prune=gr.prune_hint)
+                         out_dist=od, prune=gr.prune_hint, one_lane=gr.one_lane_hint)
 
         pc_med, _ = events_ms(per_cloud, 20)
         same = True

@@ -6,9 +6,10 @@ Larger cells certify more queries but scan more candidates.  The cell side
 is set by density feedback to give ``target_ratio * k`` candidates per
 query in the 27-cell block; the best ratio depends on the cloud size, k and
 how the points fill space.  ``tune`` times the exact query for a few ratios
-on the actual workload and records the fastest per (log2 N, k class)
-bucket in a JSON table; ``lookup_target_ratio`` reads it (falling back to
-the default).
+on the actual workload, with and without two speed hints (pruning of
+neighbour cells, one or two lanes per query = 64 or 32 queries per wave),
+and records the choice per (log2 N, k class, density class) bucket in a
+JSON table; ``lookup_*`` read it (falling back to the defaults).
 """
 from __future__ import annotations
 
@@ -74,6 +75,13 @@ def lookup_prune(n: int, k: int, dclass: int = 0) -> bool:
     return bool(e.get("prune", False)) if e else False
 
 
+def lookup_one_lane(n: int, k: int, dclass: int = 0) -> bool:
+    """Tuned FK_QUERY_ONE_LANE choice (64 instead of 32 queries per wave);
+    else off."""
+    e = _entry(n, k, dclass)
+    return bool(e.get("one_lane", False)) if e else False
+
+
 def _time_ms(fn, reps=11):
     fn()
     torch.cuda.synchronize()
@@ -86,11 +94,17 @@ def _time_ms(fn, reps=11):
     return sorted(st[i].elapsed_time(st[i + 1]) for i in range(reps))[reps // 2]
 
 
+def _label(key):
+    r, pr, one = key
+    return f"{r}{'p' if pr else ''}{'1' if one else ''}"
+
+
 def tune(xyz, offsets, k, ratios=RATIOS, save=True):
     """Time the exact query of one cloud at each target ratio, with and
-    without FK_QUERY_PRUNE (k <= 16); record the best under the cloud's
-    (size, k, density class) bucket: the largest ratio within 3 % of the
-    fastest.  Returns ((ratio, prune), {(ratio, prune): ms}, density class)."""
+    without FK_QUERY_PRUNE (k <= 16) and with one or two lanes per query
+    (FK_QUERY_ONE_LANE, k <= 32); record the best under the cloud's (size, k,
+    density class) bucket: the largest ratio within 3 % of the fastest.
+    Returns ((ratio, prune, one_lane), {key: ms}, density class)."""
     from .knn import ExactKNN, density_class
     eng = ExactKNN()
     eng.build(xyz, offsets, k, adapt=False)
@@ -100,8 +114,9 @@ def tune(xyz, offsets, k, ratios=RATIOS, save=True):
     for r in ratios:
         eng.build(xyz, offsets, k, target_ratio=r)
         for pr in ((False, True) if k <= 16 else (False,)):
-            results[(r, pr)] = _time_ms(
-                lambda: eng.grid.query(k, mode=1, pad_mode=1, prune=pr))
+            for one in ((False, True) if k <= 32 else (False,)):
+                results[(r, pr, one)] = _time_ms(
+                    lambda: eng.grid.query(k, mode=1, pad_mode=1, prune=pr, one_lane=one))
     # within 3 % of the fastest, take the largest cells: they leave fewer
     # queries to the shell expansion, which costs most where many clouds
     # share one launch (the batched kernel)
@@ -112,9 +127,9 @@ def tune(xyz, offsets, k, ratios=RATIOS, save=True):
         with _LOCK:
             t = _load()
             t[bucket(n, k, dclass)] = {
-                "target_ratio": best[0], "prune": best[1], "density_class": dclass,
-                "ms": {f"{r}{'p' if pr else ''}": round(v, 4)
-                       for (r, pr), v in results.items()}}
+                "target_ratio": best[0], "prune": best[1], "one_lane": best[2],
+                "density_class": dclass,
+                "ms": {_label(key): round(v, 4) for key, v in results.items()}}
             tmp = TABLE_PATH + ".tmp"
             with open(tmp, "w") as f:
                 json.dump(t, f, indent=1, sort_keys=True)
@@ -148,10 +163,10 @@ def main(argv=None):
             for k in (int(s) for s in args.ks.split(",")):
                 best, res, dc = tune(x, o, k)
                 print(f"{name} n={len(pts)} k={k} d={dc}: best ratio {best[0]}"
-                      f"{' prune' if best[1] else ''} ({res[best]:.4f} ms; " +
-                      ", ".join(f"{r}{'p' if pr else ''}:{m:.4f}"
-                                for (r, pr), m in sorted(res.items())) + ")",
-                      flush=True)
+                      f"{' prune' if best[1] else ''}{' one-lane' if best[2] else ''}"
+                      f" ({res[best]:.4f} ms; " +
+                      ", ".join(f"{_label(key)}:{m:.4f}" for key, m in sorted(res.items()))
+                      + ")", flush=True)
 
 
 if __name__ == "__main__":

@@ -544,6 +544,7 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.exact = p->mode == FK_MODE_EXACT;
   a.passes = p->passes ? (p->passes & 7) : 7;
   a.prune = (p->flags & FK_QUERY_PRUNE) ? 1 : 0;
+  a.one_lane = (p->flags & FK_QUERY_ONE_LANE) ? 1 : 0;
   a.scalars = g->scalars;
   a.qcnt = g->scalars + kCounters + kCntStride * g->qset;
   a.qcnt_next = nullptr;

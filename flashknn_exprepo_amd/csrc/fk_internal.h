@@ -143,6 +143,7 @@ struct QueryArgs {
   int32_t exact;
   int32_t passes;        // bit 0 reset, bit 1 first pass, bit 2 expansion
   int32_t prune;         // FK_QUERY_PRUNE: prune neighbour cells mid-scan
+  int32_t one_lane;      // FK_QUERY_ONE_LANE: one lane per query (64 per wave)
   int32_t *scalars;
   int32_t *qcnt;         // this call's counter set (Counter slots)
   int32_t *qcnt_next;    // set the first pass clears for the next call (or null)

@@ -706,7 +706,8 @@ constexpr int waves_per_block() {
   return KC <= 16 ? 4 : (KC == 32 ? 2 : 1);
 }
 
-template <int KC>
+// G lanes per query (the default per K class, or 1 with FK_QUERY_ONE_LANE).
+template <int KC, int G>
 __global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
 k_query(QueryArgs a) {
   constexpr int WPB = waves_per_block<KC>();
@@ -714,7 +715,6 @@ k_query(QueryArgs a) {
   __shared__ uint64_t sbuf[WPB * BS * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
-  constexpr int G = lanes_per_query<KC>();
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int q0 = (blk * WPB + w) * (kWave / G);
   // wave-uniform exits: past the end, or the build/attach raised an error
@@ -1203,14 +1203,24 @@ unsigned expand_blocks(int nq) {
   return (unsigned)(b < 1 ? 1 : b);
 }
 
-template <int KC>
-hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
+template <int KC, int G>
+void launch_first(const QueryArgs &a, hipStream_t s) {
   constexpr int WPB = waves_per_block<KC>();
-  constexpr int QPW = kWave / lanes_per_query<KC>();  // queries per wave
+  constexpr int QPW = kWave / G;  // queries per wave
   const int64_t waves = ((int64_t)a.nq + QPW - 1) / QPW;
   const unsigned nb = (unsigned)((waves + WPB - 1) / WPB);
-  if (a.passes & 2)
-    hipLaunchKernelGGL(k_query<KC>, dim3(nb), dim3(64 * WPB), 0, s, a);
+  hipLaunchKernelGGL((k_query<KC, G>), dim3(nb), dim3(64 * WPB), 0, s, a);
+}
+
+template <int KC>
+hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
+  constexpr int G = lanes_per_query<KC>();
+  if (a.passes & 2) {
+    if (G > 1 && a.one_lane)
+      launch_first<KC, 1>(a, s);
+    else
+      launch_first<KC, G>(a, s);
+  }
   if (a.exact && (a.passes & 4))
     hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)),
                        dim3(64 * kCoopWaves), 0, s, a);

@@ -103,6 +103,7 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
     the autotuner's choice for the clouds holding most points."""
     n = xyz.shape[0]
     g.prune_hint = bool(prune) if prune is not None else False
+    g.one_lane_hint = False
     if cell is not None:
         side = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
         side = side.expand(offsets.shape[0]).contiguous()
@@ -113,7 +114,7 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
     if not adapt or n < ADAPT_MIN_POINTS:
         return side
     if target_ratio is None:
-        from .autotune import lookup_prune, lookup_target_ratio
+        from .autotune import lookup_one_lane, lookup_prune, lookup_target_ratio
         ends = offsets.long()
         sizes = torch.diff(ends, prepend=ends.new_zeros(1)).clamp_min(1)
         mean = (g.cloud_candidates().double() / sizes.double()).cpu().tolist()
@@ -123,10 +124,12 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
         target_ratio = torch.tensor(
             [lookup_target_ratio(nc, kc, dc) for nc, kc, dc in zip(ns, ks, dcs)],
             dtype=torch.float32, device=xyz.device)
+        def vote(look):  # the choice of the clouds holding most points
+            return sum(nc if look(nc, kc, dc) else -nc
+                       for nc, kc, dc in zip(ns, ks, dcs)) > 0
         if prune is None:
-            votes = sum(nc if lookup_prune(nc, kc, dc) else -nc
-                        for nc, kc, dc in zip(ns, ks, dcs))
-            g.prune_hint = votes > 0
+            g.prune_hint = vote(lookup_prune)
+        g.one_lane_hint = vote(lookup_one_lane)
     for _ in range(ADAPT_ROUNDS if rounds is None else rounds):
         g.adapt_cell(k, side, inv, target_ratio, CANDIDATE_EXPONENT)
         g.build_xyz(xyz, offsets, inv, no_sync=no_sync)
@@ -207,7 +210,7 @@ class ExactKNN:
                              qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32),
                              no_sync=no_sync)
         out = g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, attached=attached,
-                      prune=g.prune_hint, **qkw)
+                      prune=g.prune_hint, one_lane=g.one_lane_hint, **qkw)
         g.status()  # the one host read: reports any device-side error
         return out
 

@@ -127,6 +127,10 @@ typedef struct fk_query_params {
                             current k-th distance (grids built from xyz, k
                             <= 16); faster when the 27-cell block holds many
                             more candidates than k                         */
+#define FK_QUERY_ONE_LANE 2 /* first pass with one lane (64 queries) per
+                               wave instead of two lanes per query, for
+                               k <= 32 (fk_query; the batched kernel keeps
+                               two)                                         */
 
 typedef struct fk_grid fk_grid;
 

@@ -277,11 +277,12 @@ def test_batched_k_range_errors():
         bk.grid.status()
 
 
-@pytest.mark.parametrize("k", [1, 8, 16])
+@pytest.mark.parametrize("k", [1, 8, 16, 32])
 def test_exact_prune_changes_nothing(oracle, k):
     """FK_QUERY_PRUNE (skip neighbour cells beyond the current k-th distance)
-    is a speed hint: exact results equal brute force with and without it,
-    at cell sizes from much smaller to much larger than the k-th distance."""
+    and FK_QUERY_ONE_LANE (64 instead of 32 queries per wave) are speed
+    hints: exact results equal brute force with and without them, at cell
+    sizes from much smaller to much larger than the k-th distance."""
     from flashknn_exprepo_amd import _lib as F
     from flashknn_exprepo_amd import synthetic
     rng = np.random.default_rng(22)
@@ -295,8 +296,10 @@ def test_exact_prune_changes_nothing(oracle, k):
         inv = torch.tensor([1 / cell, 1 / (2 * cell)], dtype=torch.float32, device=_dev())
         g.build_xyz(_t(xyz), _t(off), inv)
         for prune in (False, True):
-            idx, d = g.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, prune=prune)
-            _same(idx, d, ref_i, ref_d)
+            for one in (False, True):
+                idx, d = g.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1,
+                                 prune=prune, one_lane=one)
+                _same(idx, d, ref_i, ref_d)
 
 
 def test_cells_prune_matches_oracle(oracle):
