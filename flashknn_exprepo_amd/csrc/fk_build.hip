@@ -410,17 +410,49 @@ __device__ __forceinline__ int32_t lin_of(const CloudInfo &ci, int lx, int ly,
   return ci.base + (lx * ci.ny + ly) * ci.nz + lz;
 }
 
-// Linear cell id and in-cell rank (atomic counting) of every point.
-__global__ void k_count(const int4 *pt_cell, int64_t n, const CloudInfo *clouds,
-                        int32_t *counts, int32_t *pt_lin, int32_t *pt_rank,
-                        const int32_t *scalars) {
+// In-cell rank from a counting atomic, aggregated over the lanes of a wave
+// that fall in one cell: the first kAggRounds distinct cells of the wave
+// take one atomic each (coarse grids put whole waves in one cell, whose
+// per-lane atomics would serialise), the remaining lanes one atomic each.
+// Every lane of the wave calls it; lanes with act = false get 0.
+constexpr int kAggRounds = 4;
+
+__device__ __forceinline__ int32_t wave_rank(int32_t *counts, int32_t lin,
+                                             bool act) {
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(act);
+  int32_t rank = 0;
+  for (int r = 0; r < kAggRounds && pending; ++r) {
+    const int leader = __ffsll((unsigned long long)pending) - 1;
+    const int32_t llin = __shfl(lin, leader, kWave);
+    const uint64_t same = __ballot(act && lin == llin) & pending;
+    int32_t base = 0;
+    if (lane == leader) base = atomicAdd(counts + llin, (int)__popcll(same));
+    base = __shfl(base, leader, kWave);
+    if ((same >> lane) & 1)
+      rank = base + (int)__popcll(same & ((1ull << lane) - 1));
+    pending &= ~same;
+  }
+  if ((pending >> lane) & 1) rank = atomicAdd(counts + lin, 1);
+  return rank;
+}
+
+// Linear cell id and in-cell rank of every point.
+__global__ void __launch_bounds__(kBlock)
+k_count(const int4 *pt_cell, int64_t n, const CloudInfo *clouds,
+        int32_t *counts, int32_t *pt_lin, int32_t *pt_rank,
+        const int32_t *scalars) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= n || scalars[kStatus] != 0) return;
-  const int4 c = pt_cell[i];
-  const CloudInfo ci = clouds[c.w];
-  const int32_t lin = lin_of(ci, c.x - ci.minx, c.y - ci.miny, c.z - ci.minz);
-  pt_lin[i] = lin;
-  pt_rank[i] = atomicAdd(counts + lin, 1);
+  const bool act = i < n && scalars[kStatus] == 0;
+  int32_t lin = 0;
+  if (act) {
+    const int4 c = pt_cell[i];
+    const CloudInfo ci = clouds[c.w];
+    lin = lin_of(ci, c.x - ci.minx, c.y - ci.miny, c.z - ci.minz);
+    pt_lin[i] = lin;
+  }
+  const int32_t rank = wave_rank(counts, lin, act);
+  if (act) pt_rank[i] = rank;
 }
 
 // ------------------------------------------------------------------- scan
@@ -556,42 +588,53 @@ __global__ void k_qcells(const float *qxyz, const int64_t *qcell,
                          int32_t *qcounts, int32_t *q_lin, int32_t *q_rank,
                          int4 *q_cell, int32_t *scalars) {
   const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (i >= nq || scalars[kStatus] != 0) return;
-  int cl;
-  int64_t ax, ay, az;
-  if (from_xyz) {
+  bool act = i < nq && scalars[kStatus] == 0;
+  int cl = 0;
+  int64_t ax = 0, ay = 0, az = 0;
+  if (act && from_xyz) {
     cl = cloud_of(qoffsets, B, i);
-    if (cl >= B) { raise_status(scalars, FK_ERR_DOMAIN); return; }
-    const float inv = clouds[cl].inv_cell;
-    const float fx = floorf(scaled(qxyz[3 * i], inv));
-    const float fy = floorf(scaled(qxyz[3 * i + 1], inv));
-    const float fz = floorf(scaled(qxyz[3 * i + 2], inv));
-    if (!(fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
-          fabsf(fz) < (float)kCellLimit)) {
+    if (cl >= B) {
       raise_status(scalars, FK_ERR_DOMAIN);
-      return;
+      act = false;
+    } else {
+      const float inv = clouds[cl].inv_cell;
+      const float fx = floorf(scaled(qxyz[3 * i], inv));
+      const float fy = floorf(scaled(qxyz[3 * i + 1], inv));
+      const float fz = floorf(scaled(qxyz[3 * i + 2], inv));
+      if (!(fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
+            fabsf(fz) < (float)kCellLimit)) {
+        raise_status(scalars, FK_ERR_DOMAIN);
+        act = false;
+      }
+      ax = (int64_t)fx; ay = (int64_t)fy; az = (int64_t)fz;
     }
-    ax = (int64_t)fx; ay = (int64_t)fy; az = (int64_t)fz;
-  } else {
+  } else if (act) {
     const int64_t b = qbatch ? qbatch[i] : 0;
-    if (b < 0 || b >= B) { raise_status(scalars, FK_ERR_DOMAIN); return; }
-    cl = (int)b;
-    ax = qcell[3 * i] >> num_down;
-    ay = qcell[3 * i + 1] >> num_down;
-    az = qcell[3 * i + 2] >> num_down;
+    if (b < 0 || b >= B) {
+      raise_status(scalars, FK_ERR_DOMAIN);
+      act = false;
+    } else {
+      cl = (int)b;
+      ax = qcell[3 * i] >> num_down;
+      ay = qcell[3 * i + 1] >> num_down;
+      az = qcell[3 * i + 2] >> num_down;
+    }
   }
-  const CloudInfo ci = clouds[cl];
-  const int lx = sat_local(ax - ci.minx), ly = sat_local(ay - ci.miny),
-            lz = sat_local(az - ci.minz);
-  q_cell[i] = make_int4(lx, ly, lz, cl);
-  int32_t lin;
-  if (ci.nx == 0)
-    lin = scalars[kTotalCells];
-  else
-    lin = lin_of(ci, clampi(lx, 0, ci.nx - 1), clampi(ly, 0, ci.ny - 1),
-                 clampi(lz, 0, ci.nz - 1));
-  q_lin[i] = lin;
-  q_rank[i] = atomicAdd(qcounts + lin, 1);
+  int32_t lin = 0;
+  if (act) {
+    const CloudInfo ci = clouds[cl];
+    const int lx = sat_local(ax - ci.minx), ly = sat_local(ay - ci.miny),
+              lz = sat_local(az - ci.minz);
+    q_cell[i] = make_int4(lx, ly, lz, cl);
+    if (ci.nx == 0)
+      lin = scalars[kTotalCells];
+    else
+      lin = lin_of(ci, clampi(lx, 0, ci.nx - 1), clampi(ly, 0, ci.ny - 1),
+                   clampi(lz, 0, ci.nz - 1));
+    q_lin[i] = lin;
+  }
+  const int32_t rank = wave_rank(qcounts, lin, act);
+  if (act) q_rank[i] = rank;
 }
 
 __global__ void k_qscatter(const float *qxyz, int64_t nq, const int32_t *q_lin,

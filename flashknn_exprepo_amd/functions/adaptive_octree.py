@@ -34,6 +34,42 @@ from ..grid import Grid
 
 _MAX_COORD = 1 << 16
 _MAX_BATCH = 32768
+_CHUNK = 1024   # points per chunk of the per-batch minimum
+_SPARE = 4096   # spare rows that absorb the atomics of unused lanes
+
+
+def batch_min(v: torch.Tensor, batch: torch.Tensor, B: int) -> torch.Tensor:
+    """Per-batch minimum of v (int64 [N, 3]) -> int64 [B, 3], no host sync.
+
+    A scatter-min of every point onto B rows serialises N atomics on a few
+    addresses.  Here a chunk of consecutive points with one batch index
+    contributes its minimum once; points of chunks that mix batches are
+    scattered one by one, and every other scatter lane lands in one of
+    ``_SPARE`` rows that are dropped, so no row sees more than a few
+    hundred atomics when batches are stored contiguously."""
+    n = v.shape[0]
+    if B == 1:
+        return v.amin(0, keepdim=True)
+    dev = v.device
+    nc = (n + _CHUNK - 1) // _CHUNK
+    pad = nc * _CHUNK - n
+    vp = torch.cat([v, v[-1:].expand(pad, 3)]) if pad else v
+    bp = torch.cat([batch, batch[-1:].expand(pad)]) if pad else batch
+    vc = vp.view(nc, _CHUNK, 3)
+    bc = bp.view(nc, _CHUNK)
+    blo = bc.amin(1)
+    uni = blo == bc.amax(1)
+    out = torch.full((B + _SPARE, 3), torch.iinfo(torch.int64).max, dtype=torch.int64,
+                     device=dev)
+    spare_c = B + torch.arange(nc, device=dev) % _SPARE
+    idx_c = torch.where(uni, blo, spare_c)
+    out.scatter_reduce_(0, idx_c[:, None].expand(-1, 3), vc.amin(1), "amin",
+                        include_self=True)
+    spare_p = B + torch.arange(n, device=dev) % _SPARE
+    uni_p = uni[:, None].expand(nc, _CHUNK).reshape(-1)[:n]
+    idx_p = torch.where(uni_p, spare_p, batch)
+    out.scatter_reduce_(0, idx_p[:, None].expand(-1, 3), v, "amin", include_self=True)
+    return out[:B].contiguous()
 
 
 @dataclass
@@ -76,10 +112,7 @@ def build_adaptive_octree(grid_coord: torch.Tensor, batch_idx: torch.Tensor,
     torch.cuda.synchronize(grid_coord.device)
     t0 = time.perf_counter()
     B = int(batch_idx.max()) + 1
-    mins = torch.full((B, 3), torch.iinfo(torch.int64).max, dtype=torch.int64,
-                      device=grid_coord.device)
-    mins.scatter_reduce_(0, batch_idx[:, None].expand(-1, 3), grid_coord, "amin",
-                         include_self=True)
+    mins = batch_min(grid_coord, batch_idx, B)
     g = (grid_coord - mins[batch_idx]).contiguous()
     gmax = int(g.max())
     if gmax >= _MAX_COORD:
@@ -134,53 +167,118 @@ class AdaptiveNeighborhoodFlashKNN:
         self.min_candidates_factor = min_candidates_factor
         self.max_candidates_factor = max_candidates_factor
         self.last_stats: dict[str, Any] = {}
+        self._grids: dict[int, Grid] = {}
+
+    def _level_grid(self, s: int) -> Grid:
+        """Grid of level s, kept between calls (its device tables are reused)."""
+        g = self._grids.get(s)
+        if g is None:
+            g = self._grids[s] = Grid()
+        return g
 
     @torch.no_grad()
     def query(self, grid_coord: torch.Tensor, batch_idx: torch.Tensor,
               xyz: torch.Tensor | None = None, *, max_shift: int | None = None,
               batch_for_prune: int = 1, cut_radius: float = math.inf,
               return_distances: bool = False):
+        """Levels are built from the coarsest down while the selection rule
+        runs on the device; the descent stops at the first level where no
+        point may go finer (finer levels could not be selected), so the
+        finest dense boxes are built only where some point needs them.
+        Host reads: one for the sizes, one per level (does any point still
+        descend), one for the per-level query counts and one for the
+        statistics; no per-point work runs on the host."""
         K = self.num_nbr
         if K < 1 or K > F.FK_MAX_K:
             raise ValueError(f"num_nbr must be in [1, {F.FK_MAX_K}]")
-        h = build_adaptive_octree(grid_coord, batch_idx, max_shift, xyz)
-        t0 = time.perf_counter()
-        sel = select_adaptive_levels(h, K, self.min_candidates_factor,
-                                     self.max_candidates_factor)
-        torch.cuda.synchronize(grid_coord.device)
-        t_sel = (time.perf_counter() - t0) * 1e3
-        t0 = time.perf_counter()
-        n = grid_coord.shape[0]
         dev = grid_coord.device
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        if not grid_coord.is_cuda or not batch_idx.is_cuda:
+            raise ValueError("adaptive octree inputs must be HIP device tensors")
+        if grid_coord.dtype != torch.int64 or batch_idx.dtype != torch.int64:
+            raise ValueError("grid_coord and batch_idx must use torch.int64")
+        if grid_coord.ndim != 2 or grid_coord.shape[1] != 3:
+            raise ValueError("grid_coord must have shape [N, 3]")
+        if batch_idx.ndim != 1 or batch_idx.shape[0] != grid_coord.shape[0]:
+            raise ValueError("batch_idx must have shape [N]")
+        n = grid_coord.shape[0]
+        if n == 0:
+            raise ValueError("cannot construct an octree for an empty point cloud")
+        bmin, bmax = torch.aminmax(batch_idx)
+        lo = torch.stack([bmin, bmax]).cpu().tolist()
+        if lo[0] < 0 or lo[1] >= _MAX_BATCH:
+            raise ValueError("batch indices must lie in [0, 32768)")
+        B = lo[1] + 1
+        mins = batch_min(grid_coord, batch_idx, B)
+        g = (grid_coord - mins[batch_idx]).contiguous()
+        span = g.amax(0).cpu().tolist()
+        gmax = max(span)
+        if gmax >= _MAX_COORD:
+            raise ValueError("each batch's coordinate span must be smaller than 65536")
+        if max_shift is None:
+            max_shift = int(gmax).bit_length()
+        if max_shift < 0 or max_shift > 16:
+            raise ValueError("max_shift must lie in [0, 16]")
+        if n < K:
+            raise ValueError(f"point cloud contains {n} points, fewer than k={K}")
+        batch = batch_idx.contiguous()
+        pts = g.float() if xyz is None else xyz.float().contiguous()
+        lo_f, hi_f = self.min_candidates_factor * K, self.max_candidates_factor * K
+        sel = torch.full((n,), max_shift, dtype=torch.int16, device=dev)
+        counts, built = {}, {}
+        active = None
+        for s in range(max_shift, -1, -1):
+            gr = self._level_grid(s)
+            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s)
+            c = gr.point_candidates(1)
+            counts[s], built[s] = c, gr
+            if active is None:
+                active = c > hi_f
+            else:
+                use = active & (c >= lo_f)
+                sel = torch.where(use, torch.full_like(sel, s), sel)
+                active = use & (c > hi_f)
+            if s > 0 and not bool(active.any()):
+                break
+        torch.cuda.synchronize(dev)
+        t_build = (time.perf_counter() - t0) * 1e3
+        t0 = time.perf_counter()
         out_i = torch.empty((n, K), dtype=torch.int64, device=dev)
         out_d = torch.empty((n, K), dtype=torch.float32, device=dev)
-        pts = h.grid.float() if xyz is None else xyz.float()
-        levels_used = torch.unique(sel).tolist()
-        level_stats = []
-        for s in levels_used:
-            q = torch.nonzero(sel == s).flatten()
-            gr = h.grids[s]
-            gr.attach_queries(pts[q].contiguous(), qcell=h.grid[q], qbatch=h.batch[q],
+        per_level = torch.bincount(sel.long(), minlength=max_shift + 1).cpu().tolist()
+        perm = torch.argsort(sel, stable=True)
+        start, used, stat_rows = 0, [], []
+        for s in range(max_shift + 1):
+            m = per_level[s]
+            if m == 0:
+                continue
+            q = perm[start:start + m]
+            start += m
+            gr = built[s]
+            gr.attach_queries(pts[q].contiguous(), qcell=g[q], qbatch=batch[q],
                               qpad=q.to(torch.int32))
             qi, qd = gr.query(K, mode=F.FK_MODE_CELLS, radius_cells=1,
                               cut_r2=float(cut_radius) ** 2,
                               pad_mode=F.FK_PAD_PER_QUERY, attached=True)
             out_i[q] = qi
             out_d[q] = qd
-            cc = h.candidate_counts[s][q]
-            level_stats.append({"shift": int(s), "query_points": int(q.numel()),
-                                "candidate_mean": float(cc.float().mean()),
-                                "candidate_min": int(cc.min()),
-                                "candidate_max": int(cc.max())})
-        torch.cuda.synchronize(dev)
+            cc = counts[s][q].float()
+            stat_rows.append(torch.stack([cc.mean(), cc.min(), cc.max()]))
+            used.append((s, m))
+        stats = torch.stack(stat_rows).cpu().tolist() if stat_rows else []
         t_query = (time.perf_counter() - t0) * 1e3
         self.last_stats = {
-            "construction_ms": h.construction_ms, "selection_ms": t_sel,
-            "query_ms": t_query, "total_ms": h.construction_ms + t_sel + t_query,
+            "construction_ms": t_build, "selection_ms": 0.0,
+            "query_ms": t_query, "total_ms": t_build + t_query,
             "k": K, "min_candidates_factor": self.min_candidates_factor,
             "max_candidates_factor": self.max_candidates_factor,
-            "octree_levels": h.max_shift + 1, "selected_levels": level_stats,
-            "query_kernel_launches": len(levels_used),
+            "octree_levels": max_shift + 1, "levels_built": len(built),
+            "selected_levels": [
+                {"shift": int(s), "query_points": int(m), "candidate_mean": st[0],
+                 "candidate_min": int(st[1]), "candidate_max": int(st[2])}
+                for (s, m), st in zip(used, stats)],
+            "query_kernel_launches": len(used),
         }
         if return_distances:
             return out_i, out_d
