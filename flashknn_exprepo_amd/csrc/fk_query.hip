@@ -1212,12 +1212,33 @@ void launch_first(const QueryArgs &a, hipStream_t s) {
   hipLaunchKernelGGL((k_query<KC, G>), dim3(nb), dim3(64 * WPB), 0, s, a);
 }
 
+// A launch of few queries is bound by the latency of its longest scans, not
+// by throughput: below kFillWaves waves (8 per CU) the group per query is
+// widened (up to 16 lanes, each scanning every 16th candidate) until the
+// launch has that many waves.  The merged lists, and so the results, do not
+// depend on the group size.
+constexpr int64_t kFillWaves = 2048;
+
+template <int KC, int G>
+void launch_grouped(const QueryArgs &a, hipStream_t s) {
+  const int64_t lanes = (int64_t)a.nq * G;
+  if constexpr (G < 16 && buffer_slots<KC>() >= KC) {
+    if (lanes < kFillWaves * kWave) {
+      launch_grouped<KC, 2 * G>(a, s);
+      return;
+    }
+  }
+  launch_first<KC, G>(a, s);
+}
+
 template <int KC>
 hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
   constexpr int G = lanes_per_query<KC>();
   if (a.passes & 2) {
     if (G > 1 && a.one_lane)
       launch_first<KC, 1>(a, s);
+    else if constexpr (G > 1)
+      launch_grouped<KC, G>(a, s);
     else
       launch_first<KC, G>(a, s);
   }

@@ -302,6 +302,32 @@ def test_exact_prune_changes_nothing(oracle, k):
                 _same(idx, d, ref_i, ref_d)
 
 
+@pytest.mark.parametrize("n", [6000, 20000, 40000, 100000])
+def test_group_sizes_agree(oracle, n):
+    """The first pass widens the lane group of a query (2, 4, 8, 16 lanes)
+    as the launch gets smaller: at each size the exact rows equal the
+    one-lane launch's on every row and brute force on a sample, for k in
+    the three K classes that group, with and without pruning."""
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd import synthetic
+    rng = np.random.default_rng(n)
+    xyz = synthetic.subsample_to(synthetic.room(seed=48), n, rng)
+    n = len(xyz)  # the room holds 98k points
+    off = np.array([n], np.int32)
+    sample = rng.choice(n, 500, replace=False)
+    g = fk.Grid()
+    for k in (8, 16, 32):
+        side, inv = g.auto_cell(_t(xyz), _t(off), k, 1.6)
+        g.build_xyz(_t(xyz), _t(off), inv)
+        ref_i, ref_d = oracle.brute_knn(xyz, [n], xyz[sample], [500], k)
+        for prune in ((False, True) if k <= 16 else (False,)):
+            i1, d1 = g.query(k, mode=F.FK_MODE_EXACT, prune=prune, one_lane=True)
+            ig, dg = g.query(k, mode=F.FK_MODE_EXACT, prune=prune)
+            assert torch.equal(i1, ig) and torch.equal(d1, dg)
+            _same(ig[torch.from_numpy(sample).to(ig.device)],
+                  dg[torch.from_numpy(sample).to(dg.device)], ref_i, ref_d)
+
+
 def test_cells_prune_matches_oracle(oracle):
     """In the 27-cell mode on an xyz-built grid, pruning returns the same
     rows as the unpruned scan and the oracle's candidate-set kNN."""
