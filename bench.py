@@ -276,6 +276,8 @@ def main():
         feats = [torch.randn(s.shape[0], 64, device=dev) for s in stages]
         wts = [torch.randn(s.shape[0], 64, device=dev) for s in stages]
 
+        offs = [torch.tensor([s.shape[0]], dtype=torch.int32, device=dev) for s in stages]
+
         def fwd_bwd():
             loss = 0.0
             leaves = []
@@ -284,10 +286,7 @@ def main():
                 p = stages[l + 1].clone().requires_grad_(True)
                 f = feats[l + 1].clone().requires_grad_(True)
                 leaves += [q, p, f]
-                idx, d = fk.knn(p.detach(), torch.tensor([p.shape[0]], dtype=torch.int32,
-                                                         device=dev), 16,
-                                q.detach(), torch.tensor([q.shape[0]], dtype=torch.int32,
-                                                         device=dev))
+                idx, d = fk.knn(p.detach(), offs[l + 1], 16, q.detach(), offs[l])
                 out = fk.idw_interpolate(f, q, p, idx, eps=1e-8, d=d)
                 loss = loss + (out * wts[l]).sum()
             loss.backward()

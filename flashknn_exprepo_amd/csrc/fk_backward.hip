@@ -138,14 +138,14 @@ __global__ void k_dist_bw_p(const float *q, const float *p, int64_t ns,
   float ax = 0.f, ay = 0.f, az = 0.f;
   const int32_t b = start[m], end = start[m + 1];
   int32_t t = b;
-  // loads of four entries in flight; the sum still runs in entry order
-  for (; t + 4 <= end; t += 4) {
-    int32_t e[4];
-    float c[4], dx[4], dy[4], dz[4];
+  // loads of eight entries in flight; the sum still runs in entry order
+  for (; t + 8 <= end; t += 8) {
+    int32_t e[8];
+    float c[8], dx[8], dy[8], dz[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) e[u] = slots[t + u];
+    for (int u = 0; u < 8; ++u) e[u] = slots[t + u];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int64_t i = e[u] / k;
       c[u] = 2.f * g[e[u]];
       dx[u] = q[3 * i] - px;
@@ -153,7 +153,7 @@ __global__ void k_dist_bw_p(const float *q, const float *p, int64_t ns,
       dz[u] = q[3 * i + 2] - pz;
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       ax -= c[u] * dx[u];
       ay -= c[u] * dy[u];
       az -= c[u] * dz[u];
@@ -200,16 +200,16 @@ __global__ void k_idw_bw_feat(int64_t ns, int32_t C, const float *w, int32_t k,
   float acc = 0.f;
   const int32_t b = start[m], end = start[m + 1];
   int32_t s = b;
-  for (; s + 4 <= end; s += 4) {  // four gathers in flight, summed in order
-    float wv[4], gv[4];
+  for (; s + 8 <= end; s += 8) {  // eight gathers in flight, summed in order
+    float wv[8], gv[8];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
+    for (int u = 0; u < 8; ++u) {
       const int32_t e = slots[s + u];
       wv[u] = w[e];
       gv[u] = gout[(int64_t)(e / k) * C + c];
     }
 #pragma unroll
-    for (int u = 0; u < 4; ++u) acc += wv[u] * gv[u];
+    for (int u = 0; u < 8; ++u) acc += wv[u] * gv[u];
   }
   for (; s < end; ++s) {
     const int32_t e = slots[s];
@@ -259,6 +259,44 @@ __device__ __forceinline__ float wave_sum(float v) {
   return v;
 }
 
+constexpr int kGather = 8;  // neighbour rows in flight per lane (forward)
+
+// v of lane j, j wave-uniform (a scalar read, no LDS round trip)
+__device__ __forceinline__ int lane_value(int v, int j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+
+__device__ __forceinline__ int bitrev4(int v) {
+  return ((v & 1) << 3) | ((v & 2) << 1) | ((v & 4) >> 1) | ((v & 8) >> 3);
+}
+
+// One halving step of a transposed butterfly: the lanes with bit O clear
+// keep (and sum with their partner's) the lower N/2 values, the others the
+// upper N/2.
+template <int N, int O>
+__device__ __forceinline__ void fold(float (&x)[16], int lane) {
+  const bool hi = (lane & O) != 0;
+#pragma unroll
+  for (int t = 0; t < N / 2; ++t) {
+    const float send = hi ? x[t] : x[t + N / 2];
+    const float keep = hi ? x[t + N / 2] : x[t];
+    x[t] = keep + __shfl_xor(send, O, 64);
+  }
+}
+
+// Sums of x[0..15] over the 64 lanes in 17 shuffles (16 separate wave sums
+// take 96), in a fixed order: lane l returns the sum of x[bitrev4(l & 15)].
+__device__ __forceinline__ float wave_sum16(float (&x)[16], int lane) {
+  fold<16, 1>(x, lane);
+  fold<8, 2>(x, lane);
+  fold<4, 4>(x, lane);
+  fold<2, 8>(x, lane);
+  float r = x[0];
+  r += __shfl_xor(r, 16, 64);
+  r += __shfl_xor(r, 32, 64);
+  return r;
+}
+
 // Lane j < k: its neighbour m (-1 if padded or out of range) and r_j.
 __device__ __forceinline__ void interp_lane(const float *q, int64_t i,
                                             const float *p, int64_t ns,
@@ -296,11 +334,21 @@ k_interp_fwd(const float *q, int64_t nq, const float *p, int64_t ns,
   for (int c0 = 0; c0 < C; c0 += 64) {
     const int c = c0 + lane;
     float acc = 0.f;
-    for (int j = 0; j < k; ++j) {
-      const int mj = __shfl(m, j, 64);
-      const float wj = __shfl(w, j, 64);
-      if (mj >= 0 && c < C)
-        acc = __fadd_rn(acc, __fmul_rn(wj, feat[(int64_t)mj * C + c]));
+    // kGather neighbour rows in flight; the sum still runs in ascending j
+    for (int j0 = 0; j0 < k; j0 += kGather) {
+      float v[kGather], wv[kGather];
+      bool ok[kGather];
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) {
+        const int j = j0 + u;  // < 64 + kGather: lane j & 63 is only read if j < k
+        const int mj = lane_value(m, j & 63);
+        wv[u] = __int_as_float(lane_value(__float_as_int(w), j & 63));
+        ok[u] = j < k && mj >= 0;
+        v[u] = (ok[u] && c < C) ? feat[(int64_t)mj * C + c] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u)
+        if (ok[u]) acc = __fadd_rn(acc, __fmul_rn(wv[u], v[u]));
     }
     if (c < C) out[i * C + c] = acc;
   }
@@ -322,18 +370,29 @@ k_interp_bw_row(const float *q, int64_t nq, const float *p, int64_t ns,
   interp_lane(q, i, p, ns, idx, k, eps, lane, m, r);
   const float S = wave_sum(r);
   const float w = S > 0.f ? __fdiv_rn(r, S) : 0.f;
+  // a_j for 16 neighbours at a time: the 16 gathers in flight, then one
+  // transposed butterfly for the 16 dot products
   float a = 0.f;
-  for (int j = 0; j < k; ++j) {
-    const int mj = __shfl(m, j, 64);
-    float part = 0.f;
-    if (mj >= 0)
-      for (int c0 = 0; c0 < C; c0 += 64) {
-        const int c = c0 + lane;
-        if (c < C)
-          part = __fadd_rn(part, __fmul_rn(gout[i * C + c], feat[(int64_t)mj * C + c]));
+  for (int j0 = 0; j0 < k; j0 += 16) {
+    float x[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) x[u] = 0.f;
+    for (int c0 = 0; c0 < C; c0 += 64) {
+      const int c = c0 + lane;
+      const float go = c < C ? gout[i * C + c] : 0.f;
+      float v[16];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const int j = j0 + u;  // < 80: lane j & 63 is only read if j < k
+        const int mj = lane_value(m, j & 63);
+        v[u] = (j < k && mj >= 0 && c < C) ? feat[(int64_t)mj * C + c] : 0.f;
       }
-    part = wave_sum(part);
-    if (lane == j) a = part;
+#pragma unroll
+      for (int u = 0; u < 16; ++u) x[u] = __fadd_rn(x[u], __fmul_rn(go, v[u]));
+    }
+    const float red = wave_sum16(x, lane);
+    const float aj = __shfl(red, bitrev4(lane & 15), 64);
+    if (lane >= j0 && lane < j0 + 16) a = aj;
   }
   const float b = wave_sum(lane < k ? __fmul_rn(w, a) : 0.f);
   float g = 0.f;

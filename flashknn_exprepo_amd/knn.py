@@ -117,9 +117,11 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
         from .autotune import lookup_one_lane, lookup_prune, lookup_target_ratio
         ends = offsets.long()
         sizes = torch.diff(ends, prepend=ends.new_zeros(1)).clamp_min(1)
-        mean = (g.cloud_candidates().double() / sizes.double()).cpu().tolist()
+        # one host read: per-cloud sizes and first-guess candidates per query
+        both = torch.stack([sizes.double(), g.cloud_candidates().double() / sizes.double()])
+        ns, mean = both.cpu().tolist()
+        ns = [int(v) for v in ns]
         ks = k.tolist() if isinstance(k, torch.Tensor) else [int(k)] * len(mean)
-        ns = sizes.cpu().tolist()
         dcs = [density_class(mc, kc) for kc, mc in zip(ks, mean)]
         target_ratio = torch.tensor(
             [lookup_target_ratio(nc, kc, dc) for nc, kc, dc in zip(ns, ks, dcs)],
