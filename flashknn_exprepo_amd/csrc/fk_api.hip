@@ -111,6 +111,9 @@ struct fk_grid {
   // query counter sets: the set of the last plain first pass, which sets are
   // known to be zero, and the set fk_query_stats reports
   int qset = 0, stat_set = 0;
+  // exact queries of the pass the statistics cover (certified = these minus
+  // the pending count), 0 for the 27-cell mode; the same for the batched set
+  int64_t stat_nq = 0, batched_nq = 0;
   bool qclean[2] = {true, true};
   int64_t n_points = 0, n_cells = -1, n_queries = -1;
   int32_t n_clouds = 0;
@@ -574,6 +577,7 @@ int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
   FK_HIP(launch_query(a, S(stream)));
   g->qset = set;
   g->stat_set = set;
+  if (first) g->stat_nq = a.exact ? a.nq : 0;
   if (reset) g->qclean[set] = true;
   if (first) {
     g->qclean[set] = false;
@@ -587,7 +591,8 @@ int fk_query_stats(fk_grid *g, int64_t *first_pass_certified, int64_t *expanded,
   FK_CHECK(check_stream_device(g));
   FK_CHECK(read_scalars(g, S(stream)));
   const int32_t *c = g->h_pinned + kCounters + kCntStride * g->stat_set;
-  if (first_pass_certified) *first_pass_certified = c[kCntCertified];
+  const int64_t pending = c[kCntPending] < g->stat_nq ? c[kCntPending] : g->stat_nq;
+  if (first_pass_certified) *first_pass_certified = g->stat_nq - pending;
   if (expanded) *expanded = c[kCntExpanded];
   return ok();
 }
@@ -740,6 +745,8 @@ int fk_batched_query(fk_grid *g, const fk_query_params *p,
                       g->scalars, a.qcnt, s));
   FK_HIP(launch_batched(a, p->k, s));
   g->stat_set = kBatchedSet;
+  g->batched_nq = a.exact ? a.nq : 0;
+  g->stat_nq = g->batched_nq;
   return ok();
 }
 
@@ -773,6 +780,7 @@ int fk_batched_capture(fk_grid *g, const fk_query_params *p,
   }
   g->graph = graph;
   FK_HIP(hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0));
+  g->batched_nq = a.exact ? a.nq : 0;
   return ok();
 }
 
@@ -780,6 +788,7 @@ int fk_graph_replay(fk_grid *g, void *stream) {
   if (!g || !g->gexec) return fail(FK_ERR_STATE, "fk_graph_replay: nothing captured");
   FK_HIP(hipGraphLaunch(g->gexec, S(stream)));
   g->stat_set = kBatchedSet;
+  g->stat_nq = g->batched_nq;
   return ok();
 }
 

@@ -45,6 +45,11 @@ namespace {
 //   FK_HALF_MASK     bit 0/1/2: K class 16/32/64 buffers KC/2 keys (merged
 //                    with a half-cleaner) instead of KC
 //   FK_WPE_<KC>      minimum waves per SIMD requested for the first pass
+//                    (3 for K class 16 holds it to 168 VGPRs, three waves
+//                    per SIMD, at a 12-byte spill: faster than 171 VGPRs at
+//                    two)
+//   FK_SLOTS         records per lane per round of the K class 8/16 stream
+//   FK_FILL_WAVES    a launch with fewer waves widens its lane groups
 #ifndef FK_UNROLL
 #define FK_UNROLL 4
 #endif
@@ -55,7 +60,7 @@ namespace {
 #define FK_WPE_8 1
 #endif
 #ifndef FK_WPE_16
-#define FK_WPE_16 1
+#define FK_WPE_16 3
 #endif
 #ifndef FK_WPE_32
 #define FK_WPE_32 2
@@ -429,47 +434,132 @@ __device__ __forceinline__ void culled_column(const CloudInfo &ci,
   column_range(ci, cs, qc.x + dx, qc.y + dy, z0, z1, b, e);
 }
 
-// The 8 columns around the query's own (R = 1) as parallel streams: their
-// cell_start reads are issued together, then every round loads the next
-// record of each column (lane `sub` of a group takes every G-th), so the
-// rounds follow the longest column instead of the sum of all eight.
-template <int KC, int G>
-__device__ __forceinline__ void scan_ring8(TopK<KC> &t, const QueryArgs &a,
-                                           const CloudInfo &ci, int4 qc,
-                                           float4 qr, bool active, int sub,
-                                           const Cull &cu) {
-  constexpr int kRoom = buffer_slots<KC>() - 8;
-  static_assert(kRoom >= 0, "buffer too small for 8 streams");
-  int p[8], e[8];
+// The candidate ranges of one query (its own column, then the 8 around it)
+// as one stream: lane `sub` of the query's group takes every G-th record of
+// each range, the ranges are consumed one after another, kSlots consecutive
+// records per round, and a round runs on into the next range when the
+// current one ends.  A lane's rounds then follow its total record count;
+// eight parallel column streams followed the longest of its columns (about
+// twice the mean on indoor scans, several times on LiDAR rings), and the
+// rounds of a wave are the most any of its lanes needs.
+#ifndef FK_SLOTS
+#define FK_SLOTS 8
+#endif
+constexpr int kSlots = FK_SLOTS;
+constexpr int kStreamCols = 9;  // LDS table entries per lane
+
+// Table entries a kernel of K class KC needs (the stream scan serves K
+// classes 8 and 16; the others keep 1 so that the array is never empty).
+template <int KC>
+constexpr int stream_entries() {
+  return KC <= 16 ? kStreamCols : 1;
+}
+
+// The ranges live in a lane-private LDS table (entry c at tab[c * kWave]):
+// in registers they would cost 18 VGPRs and a wave per SIMD.
+template <int NC, int G>
+struct ColumnStream {
+  int2 *tab;      // this lane's table: (first record of this lane, range end)
+  uint32_t rest;  // non-empty ranges not yet started (bit c)
+  int pc, lc;     // current range: next record, records left
+  int pn, ln;     // the following non-empty range (ln = 0: none)
+
+  __device__ __forceinline__ void init(int2 *lane_tab, const int (&b)[NC],
+                                       const int (&en)[NC], int sub) {
+    static_assert(NC <= kStreamCols, "stream table too small");
+    tab = lane_tab;
+    rest = 0u;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const int s = b[c] + sub;
+      tab[c * kWave] = make_int2(s, en[c]);
+      if (s < en[c]) rest |= 1u << c;
+    }
+    pop(pc, lc);
+    pop(pn, ln);
+  }
+
+  // Next non-empty range: its first record and record count (0 when done).
+  __device__ __forceinline__ void pop(int &p, int &l) {
+    p = 0;
+    l = 0;
+    if (rest) {
+      const int c = __builtin_ctz(rest);
+      rest &= rest - 1u;
+      const int2 r = tab[c * kWave];
+      p = r.x;
+      l = (r.y - r.x + G - 1) / G;
+    }
+  }
+  __device__ __forceinline__ int addr(int u, bool &ok) const {
+    if (u < lc) {
+      ok = true;
+      return pc + u * G;
+    }
+    const int v = u - lc;
+    ok = v < ln;
+    return pn + v * G;
+  }
+
+  // After a round of kSlots records: the current range continues, or the
+  // following one (partly taken) becomes current, or both are done.
+  __device__ __forceinline__ void advance() {
+    if (lc > kSlots) {
+      pc += kSlots * G;
+      lc -= kSlots;
+      return;
+    }
+    const int used = kSlots - lc;
+    if (ln > used) {
+      pc = pn + used * G;
+      lc = ln - used;
+      pop(pn, ln);
+    } else {
+      pop(pc, lc);
+      pop(pn, ln);
+    }
+  }
+};
+
+template <int KC, int G, int NC>
+__device__ __forceinline__ void scan_stream(TopK<KC> &t, ColumnStream<NC, G> &cs,
+                                            float4 qr,
+                                            const float4 *__restrict__ srec) {
+  constexpr int kRoom = buffer_slots<KC>() - kSlots;
+  static_assert(kRoom >= 0, "buffer too small for a round");
+  for (;;) {
+    while (wave_any(cs.lc > 0) && !wave_any(t.cnt > kRoom)) {
+      float4 r[kSlots];
+      bool ok[kSlots];
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const int ad = cs.addr(u, ok[u]);
+        r[u] = srec[ok[u] ? ad : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < kSlots; ++u) {
+        const float d = sqdist(r[u].x, r[u].y, r[u].z, qr.x, qr.y, qr.z);
+        t.push(make_key(d, __float_as_uint(r[u].w)), ok[u]);
+      }
+      cs.advance();
+    }
+    if (!wave_any(cs.lc > 0)) break;
+    t.flush();
+    t.template share_threshold<G>();
+  }
+}
+
+// The 8 columns around the query's own, z within qc.z -/+ 1, trimmed by
+// the cull (all of them when it does not prune), into slots [o, o + 8).
+template <int NC>
+__device__ __forceinline__ void ring_columns(const QueryArgs &a, const CloudInfo &ci,
+                                             int4 qc, bool active, const Cull &cu,
+                                             int (&b)[NC], int (&e)[NC], int o) {
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
     const int j = c < 4 ? c : c + 1;  // 3x3 raster order without the centre
-    int b = 0, en = 0;
-    if (active) culled_column(ci, a.cell_start, cu, qc, j / 3 - 1, j % 3 - 1, b, en);
-    p[c] = b + sub;
-    e[c] = en;
-  }
-  auto left = [&]() {
-    bool any = false;
-#pragma unroll
-    for (int c = 0; c < 8; ++c) any |= p[c] < e[c];
-    return wave_any(any);
-  };
-  for (;;) {
-    while (left() && !wave_any(t.cnt > kRoom)) {
-      float4 r[8];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) r[c] = a.srec[p[c] < e[c] ? p[c] : 0];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        const float d = sqdist(r[c].x, r[c].y, r[c].z, qr.x, qr.y, qr.z);
-        t.push(make_key(d, __float_as_uint(r[c].w)), p[c] < e[c]);
-        p[c] += G;
-      }
-    }
-    if (!left()) break;
-    t.flush();
-    t.template share_threshold<G>();
+    b[o + c] = e[o + c] = 0;
+    if (active) culled_column(ci, a.cell_start, cu, qc, j / 3 - 1, j % 3 - 1, b[o + c], e[o + c]);
   }
 }
 
@@ -477,9 +567,9 @@ template <int KC, int G>
 __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
                                            float4 qr, int R, bool active,
-                                           int sub, int lane) {
+                                           int sub, int lane, int2 *lane_tab) {
   // K classes 8 and 16, R = 1: the query's own column, then the 8 neighbour
-  // columns as parallel streams.  With FK_QUERY_PRUNE an exact threshold for
+  // columns, as one stream of records.  With FK_QUERY_PRUNE an exact threshold for
   // the group after the own column (flush + mid-scan merge) prunes the
   // neighbour columns and their end cells; it pays where the 27-cell block
   // holds many more candidates than k (large cells, uneven density).  (The
@@ -487,18 +577,36 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
   // no room for the extra state.)
   if constexpr (KC <= 16 && buffer_slots<KC>() >= 16) {
     if (R == 1) {
-      int b = 0, e = 0;
+      int b0 = 0, e0 = 0;
       if (active)
-        column_range(ci, a.cell_start, qc.x, qc.y, qc.z - 1, qc.z + 1, b, e);
-      scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
-      Cull cu;
-      cu.setup(ci, qc, qr, INFINITY);
+        column_range(ci, a.cell_start, qc.x, qc.y, qc.z - 1, qc.z + 1, b0, e0);
       if (a.prune) {  // launch-uniform
+        {
+          const int b[1] = {b0}, e[1] = {e0};
+          ColumnStream<1, G> cs;
+          cs.init(lane_tab, b, e, sub);
+          scan_stream<KC, G, 1>(t, cs, qr, a.srec);
+        }
         if (wave_any(t.cnt > 0)) t.flush();
         if constexpr (G > 1) t.template group_merge_keep<G>(lane, sub);
+        Cull cu;
         cu.setup(ci, qc, qr, __uint_as_float(key_dbits(t.thr)));
+        int b[8], e[8];
+        ring_columns<8>(a, ci, qc, active, cu, b, e, 0);
+        ColumnStream<8, G> cs;
+        cs.init(lane_tab, b, e, sub);
+        scan_stream<KC, G, 8>(t, cs, qr, a.srec);
+      } else {
+        Cull cu;
+        cu.setup(ci, qc, qr, INFINITY);
+        int b[9], e[9];
+        b[0] = b0;
+        e[0] = e0;
+        ring_columns<9>(a, ci, qc, active, cu, b, e, 1);
+        ColumnStream<9, G> cs;
+        cs.init(lane_tab, b, e, sub);
+        scan_stream<KC, G, 9>(t, cs, qr, a.srec);
       }
-      scan_ring8<KC, G>(t, a, ci, qc, qr, active, sub, cu);
       return;
     }
   }
@@ -654,7 +762,7 @@ __device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
 // batched kernel.
 template <int KC, int G>
 __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
-                                         uint64_t *lane_buf) {
+                                         uint64_t *lane_buf, int2 *lane_tab) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sub = lane & (G - 1);
   const int qi = q0 + lane / G;
@@ -666,7 +774,7 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
   const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
   TopK<KC> t;
   t.init(lane_buf, k, a.cut_r2, pad_value(a, qr, qs));
-  scan_block<KC, G>(t, a, ci, qc, qr, a.R, active, sub, lane);
+  scan_block<KC, G>(t, a, ci, qc, qr, a.R, active, sub, lane, lane_tab);
   if (wave_any(t.cnt > 0)) t.flush();
   if constexpr (G > 1) t.template group_merge<G>(lane);
 
@@ -677,10 +785,11 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
     float eff = __uint_as_float(key_dbits(t.kth()));
     eff = eff < a.cut_r2 ? eff : a.cut_r2;
     done = certified(gap, eff, ci.inv_cell, umax, a.R);
-    // one count and one pending slot per query: its group's lane 0
+    // one pending slot per uncertified query: its group's lane 0.  (The
+    // certified count is the query count minus the pending one: a per-wave
+    // atomic on one shared counter would serialise every wave of the launch
+    // at the memory side, ~23 ns each on MI355X.)
     const bool lead = sub == 0;
-    const int nc = __popcll(__ballot(active && done && lead));
-    if (lane == 0 && nc) atomicAdd(a.qcnt + kCntCertified, nc);
     int slot = wave_prefix_slot(active && !done && lead, a.qcnt + kCntPending);
     if constexpr (G > 1) slot = __shfl(slot, lane & ~(G - 1), kWave);
     if (active && !done && slot >= a.nq) {
@@ -713,6 +822,7 @@ k_query(QueryArgs a) {
   constexpr int WPB = waves_per_block<KC>();
   constexpr int BS = buffer_slots<KC>();
   __shared__ uint64_t sbuf[WPB * BS * kWave];
+  __shared__ int2 stab[WPB * stream_entries<KC>() * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
@@ -720,7 +830,8 @@ k_query(QueryArgs a) {
   // wave-uniform exits: past the end, or the build/attach raised an error
   // (its tables may then be incomplete)
   if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
-  run_tile<KC, G>(a, q0, a.nq, sbuf + w * BS * kWave + lane);
+  run_tile<KC, G>(a, q0, a.nq, sbuf + w * BS * kWave + lane,
+                  stab + w * stream_entries<KC>() * kWave + lane);
 }
 
 // ------------------------------------------------- cooperative expansion
@@ -981,8 +1092,12 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
     if (lane < k) write_entry(a, o + lane, t.L[0]);
     if (lane + kWave < k) write_entry(a, o + kWave + lane, t.L[1]);
   }
-  if (lane == 0 && ncert) atomicAdd(a.qcnt + kCntCertified, ncert);
-  if (lane == 0 && nexp) atomicAdd(a.qcnt + kCntExpanded, nexp);
+  // expanded queries count as pending too: certified = queries - pending
+  (void)ncert;
+  if (lane == 0 && nexp) {
+    atomicAdd(a.qcnt + kCntPending, nexp);
+    atomicAdd(a.qcnt + kCntExpanded, nexp);
+  }
 }
 
 // ----------------------------------------------------- batched persistent
@@ -1023,8 +1138,10 @@ k_batched(QueryArgs a) {
   constexpr int BS = batched_slots<MAXKC>();
   constexpr int G = batched_lanes<MAXKC>();
   __shared__ uint64_t sbuf[WPB * BS * kWave];
+  __shared__ int2 stab[WPB * kStreamCols * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   uint64_t *lane_buf = sbuf + w * BS * kWave + lane;
+  int2 *lane_tab = stab + w * kStreamCols * kWave + lane;
   const int32_t *prefix = a.item_cloud;
   const int nitems = a.scalars[kStatus] != 0 ? 0 : prefix[a.num_clouds];
   for (;;) {
@@ -1044,13 +1161,13 @@ k_batched(QueryArgs a) {
       continue;
     }
     if (k <= 8 || MAXKC == 8)
-      run_tile<8, G>(a, q0, qe, lane_buf);
+      run_tile<8, G>(a, q0, qe, lane_buf, lane_tab);
     else if (k <= 16 || MAXKC == 16)
-      run_tile<(MAXKC < 16 ? MAXKC : 16), G>(a, q0, qe, lane_buf);
+      run_tile<(MAXKC < 16 ? MAXKC : 16), G>(a, q0, qe, lane_buf, lane_tab);
     else if (k <= 32 || MAXKC == 32)
-      run_tile<(MAXKC < 32 ? MAXKC : 32), G>(a, q0, qe, lane_buf);
+      run_tile<(MAXKC < 32 ? MAXKC : 32), G>(a, q0, qe, lane_buf, lane_tab);
     else
-      run_tile<MAXKC, G>(a, q0, qe, lane_buf);
+      run_tile<MAXKC, G>(a, q0, qe, lane_buf, lane_tab);
   }
 }
 
@@ -1098,23 +1215,42 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
 // Sum over queries of the candidate count |C(q)| of the block of
 // half-width R (the reference's traverse statistic,
 // FlashKNN/functions/FlashKnnWrapper.py:351-352).
-__global__ void k_candidate_count(const QueryArgs a, unsigned long long *total) {
+// One atomic per block (kCountBlock threads): per-wave atomics on one
+// counter serialise at the memory side.
+constexpr int kCountBlock = 1024;
+
+__device__ __forceinline__ unsigned long long query_candidates(const QueryArgs &a,
+                                                               int qi, int &cloud) {
+  const int4 qc = a.qcell[qi];
+  cloud = qc.w;
+  const CloudInfo ci = a.clouds[qc.w];
+  unsigned long long c = 0;
+  for (int dx = -a.R; dx <= a.R; ++dx)
+    for (int dy = -a.R; dy <= a.R; ++dy) {
+      int b, e;
+      column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
+                   qc.z + a.R, b, e);
+      c += (unsigned long long)(e - b);
+    }
+  return c;
+}
+
+__global__ void __launch_bounds__(kCountBlock)
+k_candidate_count(const QueryArgs a, unsigned long long *total) {
+  __shared__ unsigned long long wsum[kCountBlock / 64];
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long c = 0;
-  if (qi < a.nq && a.scalars[kStatus] == 0) {
-    const int4 qc = a.qcell[qi];
-    const CloudInfo ci = a.clouds[qc.w];
-    for (int dx = -a.R; dx <= a.R; ++dx)
-      for (int dy = -a.R; dy <= a.R; ++dy) {
-        int b, e;
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
-                     qc.z + a.R, b, e);
-        c += (unsigned long long)(e - b);
-      }
-  }
+  int cloud;
+  if (qi < a.nq && a.scalars[kStatus] == 0) c = query_candidates(a, qi, cloud);
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
-  if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(total, c);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int j = 0; j < (int)(blockDim.x >> 6); ++j) t += wsum[j];
+    if (t) atomicAdd(total, t);
+  }
 }
 
 // Candidate count |C(q)| of every query, written to its output row.
@@ -1134,33 +1270,54 @@ __global__ void k_point_candidates(const QueryArgs a, int32_t *out) {
   out[__float_as_int(a.qrec[qi].w)] = (int32_t)(c < INT_MAX ? c : INT_MAX);
 }
 
-// Per-cloud candidate sums (self queries, block half-width a.R).
-__global__ void k_cand_per_cloud(const QueryArgs a, unsigned long long *acc) {
+// Per-cloud candidate sums (self queries, block half-width a.R).  A block
+// whose queries all lie in one cloud (queries are sorted by cloud, so all
+// but the blocks at cloud boundaries) adds once; the others per wave, or per
+// lane where a wave spans clouds.
+__global__ void __launch_bounds__(kCountBlock)
+k_cand_per_cloud(const QueryArgs a, unsigned long long *acc) {
+  __shared__ unsigned long long wsum[kCountBlock / 64];
+  __shared__ int wcloud[kCountBlock / 64];
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
   const bool ok = qi < a.nq && a.scalars[kStatus] == 0;
   unsigned long long c = 0;
-  int cloud = 0;
-  if (ok) {
-    const int4 qc = a.qcell[qi];
-    cloud = qc.w;
-    const CloudInfo ci = a.clouds[cloud];
-    for (int dx = -a.R; dx <= a.R; ++dx)
-      for (int dy = -a.R; dy <= a.R; ++dy) {
-        int b, e;
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
-                     qc.z + a.R, b, e);
-        c += (unsigned long long)(e - b);
-      }
-  }
+  int cloud = -1;
+  if (ok) c = query_candidates(a, qi, cloud);
+  const int w = threadIdx.x >> 6;
   const uint64_t m = __ballot(ok);
-  if (m == 0ull) return;
-  const int c0 = __shfl(cloud, (int)__builtin_ctzll(m), kWave);
-  if (__ballot(ok && cloud != c0) == 0ull) {
+  // wave cloud: -1 empty, -2 mixed (its lanes already added), else the cloud
+  int wc = -1;
+  if (m != 0ull) {
+    const int c0 = __shfl(cloud, (int)__builtin_ctzll(m), kWave);
+    if (__ballot(ok && cloud != c0) == 0ull) {
+      wc = c0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(acc + c0, c);
-  } else if (ok && c) {
-    atomicAdd(acc + cloud, c);
+      for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, kWave);
+    } else {
+      wc = -2;
+      if (ok && c) atomicAdd(acc + cloud, c);
+    }
+  }
+  if ((threadIdx.x & (kWave - 1)) == 0) {
+    wsum[w] = c;
+    wcloud[w] = wc;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = blockDim.x >> 6;
+    int cur = -1;
+    unsigned long long run = 0;
+    for (int j = 0; j < nw; ++j) {
+      const int cj = wcloud[j];
+      if (cj < 0) continue;
+      if (cj != cur) {
+        if (cur >= 0 && run) atomicAdd(acc + cur, run);
+        cur = cj;
+        run = 0;
+      }
+      run += wsum[j];
+    }
+    if (cur >= 0 && run) atomicAdd(acc + cur, run);
   }
 }
 
@@ -1217,7 +1374,10 @@ void launch_first(const QueryArgs &a, hipStream_t s) {
 // widened (up to 16 lanes, each scanning every 16th candidate) until the
 // launch has that many waves.  The merged lists, and so the results, do not
 // depend on the group size.
-constexpr int64_t kFillWaves = 2048;
+#ifndef FK_FILL_WAVES
+#define FK_FILL_WAVES 2048
+#endif
+constexpr int64_t kFillWaves = FK_FILL_WAVES;
 
 template <int KC, int G>
 void launch_grouped(const QueryArgs &a, hipStream_t s) {
@@ -1295,8 +1455,8 @@ hipError_t launch_query(const QueryArgs &a, hipStream_t s) {
 hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
                                   hipStream_t s) {
   if (a.nq <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_candidate_count, dim3((a.nq + 255) / 256), dim3(256), 0, s,
-                     a, total);
+  hipLaunchKernelGGL(k_candidate_count, dim3((a.nq + kCountBlock - 1) / kCountBlock),
+                     dim3(kCountBlock), 0, s, a, total);
   return hipGetLastError();
 }
 
@@ -1313,8 +1473,8 @@ hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
   const int B = a.num_clouds;
   hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
   if (a.nq > 0)
-    hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + 255) / 256), dim3(256), 0, s,
-                       a, acc);
+    hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + kCountBlock - 1) / kCountBlock),
+                       dim3(kCountBlock), 0, s, a, acc);
   return hipGetLastError();
 }
 
@@ -1326,8 +1486,8 @@ hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *acc,
   const int B = a.num_clouds;
   hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
   if (a.nq > 0)
-    hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + 255) / 256), dim3(256), 0, s,
-                       a, acc);
+    hipLaunchKernelGGL(k_cand_per_cloud, dim3((a.nq + kCountBlock - 1) / kCountBlock),
+                       dim3(kCountBlock), 0, s, a, acc);
   hipLaunchKernelGGL(k_adapt, dim3((B + 255) / 256), dim3(256), 0, s, a.clouds, B,
                      acc, k_per_cloud, k, target_ratio, ratio_pc, exponent, side_io,
                      inv_io);
