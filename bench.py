@@ -153,7 +153,7 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic, "kernel": "k_query<16> (first pass)",
+                "traffic": traffic, "kernel": "k_query<16, 2> (first pass)",
                 "kernel_ms": round(p1_ms, 5), "algorithmic_bytes": bytes_q}
 
     if args.copy_mb > 0:

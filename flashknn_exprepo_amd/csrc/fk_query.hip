@@ -203,26 +203,31 @@ struct TopK {
   uint64_t *lb;     // this lane's buffer: slot s at lb[s * 64]
   int cnt;
   int k;
-  int base;         // KC - k: first real slot
-  uint64_t cutkey;
-  uint64_t padkey;  // (+inf, pad): an empty slot
-  uint64_t thr;     // accept key < thr
+  uint32_t cutbits;  // bits of the cut distance
+  uint32_t pad;      // index of an empty slot
+  uint64_t thr;      // accept key < thr
+  // (32-bit fields and derived values: the K class 16 pass runs at the edge
+  // of three waves per SIMD, where every VGPR counts)
+
+  __device__ __forceinline__ int base() const { return KC - k; }  // first real slot
+  __device__ __forceinline__ uint64_t cutkey() const { return key_of_bits(cutbits, 0u); }
+  __device__ __forceinline__ uint64_t padkey() const { return key_of_bits(kInfBits, pad); }
 
   __device__ __forceinline__ void init(uint64_t *lane_buf, int k_, float cut_r2,
-                                       uint32_t pad) {
+                                       uint32_t pad_) {
     lb = lane_buf;
     cnt = 0;
     k = k_;
-    base = KC - k_;
-    cutkey = make_key(cut_r2, 0u);
-    padkey = key_of_bits(kInfBits, pad);
+    cutbits = __float_as_uint(cut_r2);
+    pad = pad_;
 #pragma unroll
-    for (int i = 0; i < KC; ++i) top[i] = i < base ? 0ull : padkey;
-    thr = cutkey;  // (inf, pad) >= cutkey for every cut
+    for (int i = 0; i < KC; ++i) top[i] = i < base() ? 0ull : padkey();
+    thr = cutkey();  // (inf, pad) >= cutkey for every cut
   }
 
   __device__ __forceinline__ void refresh_thr() {
-    thr = top[KC - 1] < cutkey ? top[KC - 1] : cutkey;
+    const uint64_t ck = cutkey();
+    thr = top[KC - 1] < ck ? top[KC - 1] : ck;
   }
 
   // Merge the buffered keys into the sorted list (uniform work per lane).
@@ -290,7 +295,7 @@ struct TopK {
     for (int o = 1; o < G; o <<= 1) {
 #pragma unroll
       for (int j = 0; j < KC; ++j)
-        if (j >= base) lb[(j - base) * kWave] = top[j];
+        if (j >= base()) lb[(j - base()) * kWave] = top[j];
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -317,8 +322,8 @@ struct TopK {
     group_merge<G>(lane);
     if (sub != 0) {
 #pragma unroll
-      for (int i = 0; i < KC; ++i) top[i] = i < base ? 0ull : padkey;
-      thr = cutkey;
+      for (int i = 0; i < KC; ++i) top[i] = i < base() ? 0ull : padkey();
+      thr = cutkey();
     }
     share_threshold<G>();
   }
@@ -738,10 +743,10 @@ __device__ __forceinline__ void write_row(const QueryArgs &a, const TopK<KC> &t,
                                           int64_t o, int sub) {
   // real entry j lives in register slot base + j; lane `sub` of the group
   // writes the entries j with j % G == sub
-  const int64_t ob = o - t.base;
+  const int64_t ob = o - t.base();
 #pragma unroll
   for (int j = 0; j < KC; ++j)
-    if (j >= t.base && ((j - t.base) & (G - 1)) == sub) write_entry(a, ob + j, t.top[j]);
+    if (j >= t.base() && ((j - t.base()) & (G - 1)) == sub) write_entry(a, ob + j, t.top[j]);
 }
 
 __device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
@@ -797,10 +802,10 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
       if (lead) raise_status(a.scalars, FK_ERR_STATE);
     } else if (active && !done) {
       if (lead) a.pend_q[slot] = qs;
-      const int64_t pb = (int64_t)slot * a.k - t.base;
+      const int64_t pb = (int64_t)slot * a.k - t.base();
 #pragma unroll
       for (int j = 0; j < KC; ++j)
-        if (j >= t.base && ((j - t.base) & (G - 1)) == sub)
+        if (j >= t.base() && ((j - t.base()) & (G - 1)) == sub)
           a.pend_keys[pb + j] = t.top[j];
     }
   }
