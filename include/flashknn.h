@@ -130,7 +130,10 @@ typedef struct fk_query_params {
 #define FK_QUERY_ONE_LANE 2 /* first pass with one lane (64 queries) per
                                wave instead of two lanes per query, for
                                k <= 32 (fk_query; the batched kernel keeps
-                               two)                                         */
+                               two).  Without it, a launch too small to
+                               fill the GPU (under 2,048 waves) widens
+                               its groups to 4, 8 or 16 lanes per query;
+                               results never depend on the group size     */
 
 typedef struct fk_grid fk_grid;
 
@@ -232,7 +235,9 @@ int fk_grid_tables(fk_grid *grid, const void **sorted_records,
 int fk_query(fk_grid *grid, const fk_query_params *params, void *out_idx,
              float *out_dist, void *stream);
 /* Statistics of the last FK_MODE_EXACT query: queries certified by the
- * first pass, and those resolved by shell expansion.  Synchronises. */
+ * first pass (the query count minus those left pending), and those
+ * resolved by shell expansion; 0 and 0 after a 27-cell query.
+ * Synchronises. */
 int fk_query_stats(fk_grid *grid, int64_t *first_pass_certified,
                    int64_t *expanded, void *stream);
 /* Sum over the query set of the number of candidates in the block of
