@@ -946,9 +946,11 @@ struct CoopList {
 
 // Columns of the shell Ro < Chebyshev distance <= Rn, split over the lanes
 // (Ro = -1: the whole block of half-width Rn).
+constexpr int kShellBatch = 8;  // column segments looked up together per lane
+
 template <int P>
-__device__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &ci,
-                           int4 qc, float4 qr, int Ro, int Rn, int lane) {
+__device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &ci,
+                           int4 qc, float4 qr, int Ro, int Rn, int lane, int2 *stab) {
   const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
   const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
   const long long wy = y1 >= y0 ? (long long)(y1 - y0 + 1) : 0;
@@ -960,27 +962,47 @@ __device__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &
   const int zlo = zc(zq - Rn), zhi = zc(zq + Rn);
   const int zin_lo = zc(zq - Ro - 1), zin_hi = zc(zq + Ro + 1);
   long long s = lane;  // this lane's next column
-  int seg = 0;         // inner columns have two z segments
   int p = 0, e = 0;
-  auto advance = [&]() {
-    while (p >= e && s < ncol) {
-      const int dx = x0 + (int)(s / wy), dy = y0 + (int)(s % wy);
-      const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
-      if (inner) {
-        if (seg == 0)
-          column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zin_lo, p, e);
-        else
-          column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zin_hi, zhi, p, e);
-        if (seg == 0) {
-          seg = 1;
-        } else {
-          seg = 0;
-          s += kWave;
-        }
-      } else {
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, zhi, p, e);
-        s += kWave;
+  // The lane's columns are looked up kShellBatch at a time: their
+  // cell-table reads go out together and the non-empty ranges (an inner
+  // column has two z segments) are queued in the lane's LDS table.  (One at
+  // a time, a sparse query's large shell -- hundreds of mostly empty columns
+  // per lane -- was a chain of dependent table reads.)
+  int2 *tab = stab;
+  int qn = 0, qi = 0;  // queued ranges, next to take
+  auto fill = [&]() {
+    int b[2 * kShellBatch], en[2 * kShellBatch];
+#pragma unroll
+    for (int j = 0; j < kShellBatch; ++j) {
+      const long long sj = s + (long long)j * kWave;
+      b[2 * j] = en[2 * j] = b[2 * j + 1] = en[2 * j + 1] = 0;
+      if (sj < ncol) {
+        const int dx = x0 + (int)(sj / wy), dy = y0 + (int)(sj % wy);
+        const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
+        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, inner ? zin_lo : zhi,
+                     b[2 * j], en[2 * j]);
+        if (inner)
+          column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zin_hi, zhi,
+                       b[2 * j + 1], en[2 * j + 1]);
       }
+    }
+    s += (long long)kShellBatch * kWave;
+    qn = 0;
+    qi = 0;
+#pragma unroll
+    for (int j = 0; j < 2 * kShellBatch; ++j)
+      if (b[j] < en[j]) tab[(qn++) * kWave] = make_int2(b[j], en[j]);
+  };
+  auto advance = [&]() {
+    while (p >= e) {
+      if (qi >= qn) {
+        if (s >= ncol) return;
+        fill();
+        continue;
+      }
+      const int2 r = tab[(qi++) * kWave];
+      p = r.x;
+      e = r.y;
     }
   };
   advance();
@@ -1016,7 +1038,7 @@ __device__ __forceinline__ float coop_eff(const CoopList<P> &t, float cut_r2) {
 template <int P>
 __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc, float4 qr,
-                                           int Ro, int lane) {
+                                           int Ro, int lane, int2 *stab) {
   const int Rcover = cover_radius(ci, qc);
   for (;;) {  // wave-uniform
     double umax;
@@ -1024,14 +1046,16 @@ __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
     if (certified(gap, coop_eff(t, a.cut_r2), ci.inv_cell, umax, Ro) || Ro >= Rcover)
       return Ro;
     const int Rn = next_radius(ci, qc, qr, Ro, coop_eff(t, a.cut_r2), Rcover);
-    coop_shell<P>(t, a, ci, qc, qr, Ro, Rn, lane);
+    coop_shell<P>(t, a, ci, qc, qr, Ro, Rn, lane, stab);
     Ro = Rn;
   }
 }
 
 __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
+  __shared__ int2 ctab[kCoopWaves * 2 * kShellBatch * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  int2 *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
   const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq);
   int nexp = 0;
   // static striding over the pending list (a shared work counter measured
@@ -1054,8 +1078,8 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
     const int Rcover = cover_radius(ci, qc);
     if (a.R < Rcover) {
       const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(t, a.cut_r2), Rcover);
-      coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane);
-      coop_expand<1>(t, a, ci, qc, qr, Rn, lane);
+      coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane, lane_tab);
+      coop_expand<1>(t, a, ci, qc, qr, Rn, lane, lane_tab);
     }
     ++nexp;
     if (lane < k)
@@ -1069,7 +1093,9 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
 // grows it until the query certifies.
 __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
+  __shared__ int2 ctab[kCoopWaves * 2 * kShellBatch * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  int2 *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int nq = a.scalars[kStatus] != 0 ? 0 : a.nq;
   const int nwaves = gridDim.x * kCoopWaves;
@@ -1088,9 +1114,9 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
     t.L[0] = lane < k ? padkey : kMaxKey;
     t.L[1] = lane + kWave < k ? padkey : kMaxKey;
     t.refresh(lane);
-    coop_shell<2>(t, a, ci, qc, qr, -1, a.R, lane);
+    coop_shell<2>(t, a, ci, qc, qr, -1, a.R, lane, lane_tab);
     if (a.exact) {
-      const int Ro = coop_expand<2>(t, a, ci, qc, qr, a.R, lane);
+      const int Ro = coop_expand<2>(t, a, ci, qc, qr, a.R, lane, lane_tab);
       if (Ro > a.R) ++nexp; else ++ncert;
     }
     const int64_t o = out_base(a, ci, qc.w, __float_as_int(qr.w), k);
@@ -1359,8 +1385,16 @@ __global__ void k_reset_query_counters(int32_t *qcnt) {
 }
 
 // Fixed grid (graph-capturable): pending queries are read on the device.
+// Up to kExpandWaves waves, static striding over the pending list.  (One
+// wave per pending query, 16,384 waves, measured slower: LiDAR K=16 query
+// 179 -> 212 us.)
+#ifndef FK_EXPAND_WAVES
+#define FK_EXPAND_WAVES 4096
+#endif
+constexpr int kExpandWaves = FK_EXPAND_WAVES;
+
 unsigned expand_blocks(int nq) {
-  const int waves = nq < 4096 ? nq : 4096;
+  const int waves = nq < kExpandWaves ? nq : kExpandWaves;
   const int b = (waves + kCoopWaves - 1) / kCoopWaves;
   return (unsigned)(b < 1 ? 1 : b);
 }

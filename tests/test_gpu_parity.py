@@ -636,6 +636,22 @@ def test_exact_sparse_clusters(oracle):
     _same(idx, d, ref_i, ref_d)
 
 
+@pytest.mark.parametrize("k", [8, 16, 100])
+def test_exact_culled_shells_far_offset(oracle, k):
+    """Expansion shells are culled against the current k-th distance: dense
+    blobs plus sparse points far from them, the cloud shifted thousands of
+    metres from the origin (large scaled coordinates), fine cells so that the
+    sparse queries grow shells of many cells; exact against brute force."""
+    rng = np.random.default_rng(29)
+    blobs = [rng.normal(c, 0.05, (400, 3)) for c in rng.uniform(-3, 3, (6, 3))]
+    sparse = rng.uniform(-12, 12, (300, 3))
+    s = (np.concatenate(blobs + [sparse]) + np.array([4096.0, -2500.0, 731.0])).astype(np.float32)
+    ref_i, ref_d = oracle.brute_knn(s, [len(s)], s, [len(s)], k)
+    for cell in (0.05, 0.2):
+        idx, d = fk.knn(_t(s), _t(np.array([len(s)], np.int32)), k, cell=torch.tensor([cell]))
+        _same(idx, d, ref_i, ref_d)
+
+
 @pytest.mark.parametrize("k", [8, 16])
 def test_adaptive_neighborhood_lidar(oracle, k):
     from flashknn_exprepo_amd import synthetic
