@@ -453,6 +453,19 @@ __device__ __forceinline__ void culled_column(const CloudInfo &ci,
 constexpr int kSlots = FK_SLOTS;
 constexpr int kStreamCols = 9;  // LDS table entries per lane
 
+// A record range [b, e) in one 64-bit LDS word of a key buffer, moved by
+// byte copies (so the words may hold keys at other times without any
+// type-based aliasing assumption).
+__device__ __forceinline__ void put_range(uint64_t *slot, int b, int e) {
+  const int2 r = make_int2(b, e);
+  __builtin_memcpy(slot, &r, sizeof(r));
+}
+__device__ __forceinline__ int2 get_range(const uint64_t *slot) {
+  int2 r;
+  __builtin_memcpy(&r, slot, sizeof(r));
+  return r;
+}
+
 // Table entries a kernel of K class KC needs (the stream scan serves K
 // classes 8 and 16; the others keep 1 so that the array is never empty).
 template <int KC>
@@ -464,12 +477,12 @@ constexpr int stream_entries() {
 // in registers they would cost 18 VGPRs and a wave per SIMD.
 template <int NC, int G>
 struct ColumnStream {
-  int2 *tab;      // this lane's table: (first record of this lane, range end)
+  uint64_t *tab;  // this lane's table: (first record of this lane, range end)
   uint32_t rest;  // non-empty ranges not yet started (bit c)
   int pc, lc;     // current range: next record, records left
   int pn, ln;     // the following non-empty range (ln = 0: none)
 
-  __device__ __forceinline__ void init(int2 *lane_tab, const int (&b)[NC],
+  __device__ __forceinline__ void init(uint64_t *lane_tab, const int (&b)[NC],
                                        const int (&en)[NC], int sub) {
     static_assert(NC <= kStreamCols, "stream table too small");
     tab = lane_tab;
@@ -477,7 +490,7 @@ struct ColumnStream {
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
       const int s = b[c] + sub;
-      tab[c * kWave] = make_int2(s, en[c]);
+      put_range(tab + c * kWave, s, en[c]);
       if (s < en[c]) rest |= 1u << c;
     }
     pop(pc, lc);
@@ -491,7 +504,7 @@ struct ColumnStream {
     if (rest) {
       const int c = __builtin_ctz(rest);
       rest &= rest - 1u;
-      const int2 r = tab[c * kWave];
+      const int2 r = get_range(tab + c * kWave);
       p = r.x;
       l = (r.y - r.x + G - 1) / G;
     }
@@ -572,7 +585,7 @@ template <int KC, int G>
 __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
                                            float4 qr, int R, bool active,
-                                           int sub, int lane, int2 *lane_tab) {
+                                           int sub, int lane, uint64_t *lane_tab) {
   // K classes 8 and 16, R = 1: the query's own column, then the 8 neighbour
   // columns, as one stream of records.  With FK_QUERY_PRUNE an exact threshold for
   // the group after the own column (flush + mid-scan merge) prunes the
@@ -767,7 +780,7 @@ __device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
 // batched kernel.
 template <int KC, int G>
 __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
-                                         uint64_t *lane_buf, int2 *lane_tab) {
+                                         uint64_t *lane_buf, uint64_t *lane_tab) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sub = lane & (G - 1);
   const int qi = q0 + lane / G;
@@ -827,7 +840,7 @@ k_query(QueryArgs a) {
   constexpr int WPB = waves_per_block<KC>();
   constexpr int BS = buffer_slots<KC>();
   __shared__ uint64_t sbuf[WPB * BS * kWave];
-  __shared__ int2 stab[WPB * stream_entries<KC>() * kWave];
+  __shared__ uint64_t stab[WPB * stream_entries<KC>() * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   const int blk = xcd_remap(blockIdx.x, gridDim.x);
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
@@ -950,7 +963,7 @@ constexpr int kShellBatch = 8;  // column segments looked up together per lane
 
 template <int P>
 __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &ci,
-                           int4 qc, float4 qr, int Ro, int Rn, int lane, int2 *stab) {
+                           int4 qc, float4 qr, int Ro, int Rn, int lane, uint64_t *stab) {
   const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
   const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
   const long long wy = y1 >= y0 ? (long long)(y1 - y0 + 1) : 0;
@@ -968,7 +981,7 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
   // column has two z segments) are queued in the lane's LDS table.  (One at
   // a time, a sparse query's large shell -- hundreds of mostly empty columns
   // per lane -- was a chain of dependent table reads.)
-  int2 *tab = stab;
+  uint64_t *tab = stab;
   int qn = 0, qi = 0;  // queued ranges, next to take
   auto fill = [&]() {
     int b[2 * kShellBatch], en[2 * kShellBatch];
@@ -991,7 +1004,7 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
     qi = 0;
 #pragma unroll
     for (int j = 0; j < 2 * kShellBatch; ++j)
-      if (b[j] < en[j]) tab[(qn++) * kWave] = make_int2(b[j], en[j]);
+      if (b[j] < en[j]) put_range(tab + (qn++) * kWave, b[j], en[j]);
   };
   auto advance = [&]() {
     while (p >= e) {
@@ -1000,7 +1013,7 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
         fill();
         continue;
       }
-      const int2 r = tab[(qi++) * kWave];
+      const int2 r = get_range(tab + (qi++) * kWave);
       p = r.x;
       e = r.y;
     }
@@ -1038,7 +1051,7 @@ __device__ __forceinline__ float coop_eff(const CoopList<P> &t, float cut_r2) {
 template <int P>
 __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc, float4 qr,
-                                           int Ro, int lane, int2 *stab) {
+                                           int Ro, int lane, uint64_t *stab) {
   const int Rcover = cover_radius(ci, qc);
   for (;;) {  // wave-uniform
     double umax;
@@ -1053,9 +1066,9 @@ __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
 
 __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
-  __shared__ int2 ctab[kCoopWaves * 2 * kShellBatch * kWave];
+  __shared__ uint64_t ctab[kCoopWaves * 2 * kShellBatch * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  int2 *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
+  uint64_t *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
   const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq);
   int nexp = 0;
   // static striding over the pending list (a shared work counter measured
@@ -1093,9 +1106,9 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
 // grows it until the query certifies.
 __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
-  __shared__ int2 ctab[kCoopWaves * 2 * kShellBatch * kWave];
+  __shared__ uint64_t ctab[kCoopWaves * 2 * kShellBatch * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  int2 *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
+  uint64_t *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int nq = a.scalars[kStatus] != 0 ? 0 : a.nq;
   const int nwaves = gridDim.x * kCoopWaves;
@@ -1168,11 +1181,17 @@ k_batched(QueryArgs a) {
   constexpr int WPB = waves_per_block<MAXKC>();
   constexpr int BS = batched_slots<MAXKC>();
   constexpr int G = batched_lanes<MAXKC>();
+  // The column table of the K class 8/16 tiles: the tail of the lane's
+  // buffer when it is long enough (those tiles use its first 16 slots; the
+  // K class 32/64 tiles, which use all of it, take no table), else its own
+  // array.  (A separate table cut the K class 32 kernel to three blocks per
+  // CU.)
+  constexpr bool kTabInBuf = BS >= 16 + kStreamCols;
   __shared__ uint64_t sbuf[WPB * BS * kWave];
-  __shared__ int2 stab[WPB * kStreamCols * kWave];
+  __shared__ uint64_t stab[kTabInBuf ? 1 : WPB * kStreamCols * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   uint64_t *lane_buf = sbuf + w * BS * kWave + lane;
-  int2 *lane_tab = stab + w * kStreamCols * kWave + lane;
+  uint64_t *lane_tab = kTabInBuf ? lane_buf + 16 * kWave : stab + w * kStreamCols * kWave + lane;
   const int32_t *prefix = a.item_cloud;
   const int nitems = a.scalars[kStatus] != 0 ? 0 : prefix[a.num_clouds];
   for (;;) {
