@@ -681,9 +681,14 @@ __device__ __forceinline__ bool certified(double gap, float eff, float inv,
 }
 
 // Smallest block half-width that can certify `eff`, at least Ro + 1.
+// Without a k-th distance yet (found < k), the block grows by the factor
+// that would hold k points if their count grew like the area of a surface,
+// sqrt(k / found), between 1.25 and the doubling 2 + 1/Ro (found = 0:
+// doubling): scanned-out sparse surroundings (LiDAR's far range) would
+// otherwise double into shells of thousands of empty columns.
 __device__ __forceinline__ int next_radius(const CloudInfo &ci, int4 qc,
                                            float4 qr, int Ro, float eff,
-                                           int Rcover) {
+                                           int Rcover, int found, int k) {
   int Rn;
   if (eff < INFINITY) {
     const float inv = ci.inv_cell;
@@ -701,7 +706,12 @@ __device__ __forceinline__ int next_radius(const CloudInfo &ci, int4 qc,
     }
     Rn = need > (double)Rcover ? Rcover : (int)need;
   } else {
-    const long long r2 = 2ll * Ro + 1;
+    long long r2 = 2ll * Ro + 1;
+    if (found > 0 && Ro > 0) {
+      const float g = fminf(fmaxf(sqrtf((float)k / (float)found), 1.25f), 2.f);
+      const long long rg = (long long)ceilf(g * (float)Ro);
+      r2 = rg < r2 ? rg : r2;
+    }
     Rn = r2 > Rcover ? Rcover : (int)r2;
   }
   if (Rn > Rcover) Rn = Rcover;
@@ -1039,6 +1049,16 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
   if (t.cnt > 0) t.flush(lane);
 }
 
+// Real keys in a cooperative list (wave-uniform).
+template <int P>
+__device__ __forceinline__ int coop_found(const CoopList<P> &t, int lane) {
+  int n = 0;
+#pragma unroll
+  for (int p = 0; p < P; ++p)
+    n += __popcll(__ballot(p * kWave + lane < t.k && key_dbits(t.L[p]) != kInfBits));
+  return n;
+}
+
 // min(k-th distance, cut) of a cooperative list.
 template <int P>
 __device__ __forceinline__ float coop_eff(const CoopList<P> &t, float cut_r2) {
@@ -1058,7 +1078,8 @@ __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
     const double gap = cert_gap(ci, qc, qr, Ro, umax);
     if (certified(gap, coop_eff(t, a.cut_r2), ci.inv_cell, umax, Ro) || Ro >= Rcover)
       return Ro;
-    const int Rn = next_radius(ci, qc, qr, Ro, coop_eff(t, a.cut_r2), Rcover);
+    const int Rn = next_radius(ci, qc, qr, Ro, coop_eff(t, a.cut_r2), Rcover,
+                               coop_found(t, lane), t.k);
     coop_shell<P>(t, a, ci, qc, qr, Ro, Rn, lane, stab);
     Ro = Rn;
   }
@@ -1090,7 +1111,8 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
     // the first pass left it uncertified at half-width a.R
     const int Rcover = cover_radius(ci, qc);
     if (a.R < Rcover) {
-      const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(t, a.cut_r2), Rcover);
+      const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(t, a.cut_r2), Rcover,
+                                 coop_found(t, lane), k);
       coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane, lane_tab);
       coop_expand<1>(t, a, ci, qc, qr, Rn, lane, lane_tab);
     }
