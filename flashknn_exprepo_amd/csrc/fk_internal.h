@@ -25,11 +25,12 @@ enum Scalar : int {
 // One set of query counters.  Plain queries alternate between sets 0 and 1
 // (each first pass clears the other set for the next call, so no separate
 // reset launch is needed); batched launches use set 2, cleared by k_items.
+// (The certified count is not kept: it is the query count minus the
+// pending one.)
 enum Counter : int {
   kCntPending = 0,    // exact mode: queries left uncertified by pass 1
-  kCntCertified = 1,  // exact mode: certified by pass 1
   kCntExpanded = 2,   // exact mode: resolved by shell expansion
-  kCntStride = 4      // slots per set (one spare)
+  kCntStride = 4      // slots per set (slots 1 and 3 spare)
 };
 constexpr int kBatchedSet = 2;
 
