@@ -157,10 +157,19 @@ class ExactKNN:
         self._cap = (n2, B2, q2)
 
     def build(self, xyz, offsets, k, *, cell=None, scale=None, target_ratio=None,
-              adapt=True, nq=0, no_sync=True, prune=None):
+              adapt=True, nq=0, no_sync=None, prune=None):
         """Build the support grid; returns the per-cloud cell side.  The
-        FK_QUERY_PRUNE choice for queries on it is ``self.grid.prune_hint``."""
+        FK_QUERY_PRUNE choice for queries on it is ``self.grid.prune_hint``.
+
+        ``no_sync`` (default: only with automatic cell sizes) builds into
+        reserved capacity without a host read.  Automatic sizes keep every
+        box below the reserve (at most ~64 cells per point); a caller's
+        ``cell`` may not, so by default such a build sizes its box exactly.
+        A no-sync build that overflows is reported by the next synchronising
+        call (``grid.status()``), and queries on it write nothing."""
         F.require_device(xyz, "xyz")
+        if no_sync is None:
+            no_sync = cell is None
         xyz = xyz.float().contiguous()
         offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
         if no_sync:

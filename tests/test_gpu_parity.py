@@ -636,6 +636,20 @@ def test_exact_sparse_clusters(oracle):
     _same(idx, d, ref_i, ref_d)
 
 
+def test_exact_build_explicit_cell_beyond_reserve(oracle):
+    """ExactKNN.build with a caller's cell whose dense box (100^3 cells)
+    exceeds the no-sync reserve of 3,000 points: the build sizes its box
+    exactly by default, so querying the grid directly is exact."""
+    from flashknn_exprepo_amd import _lib as F
+    s = (np.random.default_rng(30).random((3000, 3)) * 20).astype(np.float32)
+    ref_i, ref_d = oracle.brute_knn(s, [3000], s, [3000], 8)
+    eng = fk.ExactKNN()
+    eng.build(_t(s), _t(np.array([3000], np.int32)), 8, cell=torch.tensor([0.2]))
+    idx, d = eng.grid.query(8, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
+    eng.grid.status()
+    _same(idx, d, ref_i, ref_d)
+
+
 @pytest.mark.parametrize("k", [8, 16, 100])
 def test_exact_culled_shells_far_offset(oracle, k):
     """Expansion shells are culled against the current k-th distance: dense
