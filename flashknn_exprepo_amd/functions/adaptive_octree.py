@@ -34,6 +34,8 @@ from ..grid import Grid
 
 _MAX_COORD = 1 << 16
 _MAX_BATCH = 32768
+_RESERVE_CAP = 1 << 28   # largest dense box a level reserves (cells)
+_SYNC_CELLS = 1 << 21    # boxes above this are built only if some point needs them
 _CHUNK = 1024   # points per chunk of the per-batch minimum
 _SPARE = 4096   # spare rows that absorb the atomics of unused lanes
 
@@ -228,9 +230,19 @@ class AdaptiveNeighborhoodFlashKNN:
         sel = torch.full((n,), max_shift, dtype=torch.int16, device=dev)
         counts, built = {}, {}
         active = None
+
+        def box_cells(s):
+            # cells of all per-batch boxes at level s: at most B boxes of
+            # (span >> s) + 1 cells per axis (shifted coordinates start at 0)
+            return B * math.prod((int(v) >> s) + 1 for v in span)
+
         for s in range(max_shift, -1, -1):
             gr = self._level_grid(s)
-            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s)
+            cap = box_cells(s) + 64
+            no_sync = cap <= _RESERVE_CAP
+            if no_sync:  # sized from the host-side span: no read-back
+                gr.reserve(n, cap, B, n)
+            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
             c = gr.point_candidates(1)
             counts[s], built[s] = c, gr
             if active is None:
@@ -239,34 +251,61 @@ class AdaptiveNeighborhoodFlashKNN:
                 use = active & (c >= lo_f)
                 sel = torch.where(use, torch.full_like(sel, s), sel)
                 active = use & (c > hi_f)
-            if s > 0 and not bool(active.any()):
+            # a finer level no point can select changes nothing; read the
+            # mask back only before a box large enough to matter
+            if s > 0 and box_cells(s - 1) > _SYNC_CELLS and not bool(active.any()):
                 break
         torch.cuda.synchronize(dev)
         t_build = (time.perf_counter() - t0) * 1e3
         t0 = time.perf_counter()
-        out_i = torch.empty((n, K), dtype=torch.int64, device=dev)
-        out_d = torch.empty((n, K), dtype=torch.float32, device=dev)
-        per_level = torch.bincount(sel.long(), minlength=max_shift + 1).cpu().tolist()
+        L = max_shift + 1
+        sel_l = sel.long()
+        # points ordered by selected level: each level's queries are one
+        # contiguous slice of the reordered inputs and outputs
         perm = torch.argsort(sel, stable=True)
-        start, used, stat_rows = 0, [], []
-        for s in range(max_shift + 1):
+        pts_p, g_p, batch_p = pts[perm], g[perm], batch[perm]
+        perm32 = perm.to(torch.int32)
+        # per-level statistics of the selected level's candidate count from one
+        # sort by (level, count): run bounds by searchsorted, sums by a prefix
+        # sum, min/max at the ends of each run (a weighted bincount over a few
+        # levels funnels every point into a handful of atomics: 0.2 ms)
+        zero = torch.zeros(n, dtype=torch.int32, device=dev)
+        c_all = torch.stack([counts.get(s, zero) for s in range(L)])
+        c_sel = c_all.gather(0, sel_l[None]).squeeze(0).long()
+        keys = torch.sort(sel_l * (1 << 32) + c_sel).values
+        cs = keys & 0xFFFFFFFF
+        bounds = torch.searchsorted(keys, torch.arange(L + 1, device=dev) * (1 << 32))
+        starts, ends = bounds[:-1], bounds[1:]
+        per_level_d = ends - starts
+        csum = torch.cat([cs.new_zeros(1), torch.cumsum(cs, 0)])
+        sums = csum[ends] - csum[starts]
+        lo_c = cs[starts.clamp(max=n - 1)]
+        hi_c = cs[(ends - 1).clamp(min=0)]
+        host = torch.stack([per_level_d, sums, lo_c, hi_c]).cpu()
+        per_level = [int(v) for v in host[0].tolist()]
+        out_i_p = torch.empty((n, K), dtype=torch.int64, device=dev)
+        out_d_p = torch.empty((n, K), dtype=torch.float32, device=dev)
+        start, used, stats = 0, [], []
+        for s in range(L):
             m = per_level[s]
             if m == 0:
                 continue
-            q = perm[start:start + m]
+            sl = slice(start, start + m)
             start += m
             gr = built[s]
-            gr.attach_queries(pts[q].contiguous(), qcell=g[q], qbatch=batch[q],
-                              qpad=q.to(torch.int32))
-            qi, qd = gr.query(K, mode=F.FK_MODE_CELLS, radius_cells=1,
-                              cut_r2=float(cut_radius) ** 2,
-                              pad_mode=F.FK_PAD_PER_QUERY, attached=True)
-            out_i[q] = qi
-            out_d[q] = qd
-            cc = counts[s][q].float()
-            stat_rows.append(torch.stack([cc.mean(), cc.min(), cc.max()]))
+            gr.attach_queries(pts_p[sl], qcell=g_p[sl], qbatch=batch_p[sl],
+                              qpad=perm32[sl], no_sync=True)
+            gr.query(K, mode=F.FK_MODE_CELLS, radius_cells=1, cut_r2=float(cut_radius) ** 2,
+                     pad_mode=F.FK_PAD_PER_QUERY, attached=True,
+                     out_idx=out_i_p[sl], out_dist=out_d_p[sl])
             used.append((s, m))
-        stats = torch.stack(stat_rows).cpu().tolist() if stat_rows else []
+            stats.append((float(host[1][s]) / m, int(host[2][s]), int(host[3][s])))
+        out_i = torch.empty_like(out_i_p)
+        out_d = torch.empty_like(out_d_p)
+        out_i[perm] = out_i_p
+        out_d[perm] = out_d_p
+        for gr in built.values():  # device errors of the no-sync builds and queries
+            gr.status()
         t_query = (time.perf_counter() - t0) * 1e3
         self.last_stats = {
             "construction_ms": t_build, "selection_ms": 0.0,
