@@ -236,24 +236,43 @@ class AdaptiveNeighborhoodFlashKNN:
             # (span >> s) + 1 cells per axis (shifted coordinates start at 0)
             return B * math.prod((int(v) >> s) + 1 for v in span)
 
+        # The coarse levels (small boxes) form one grid: the points once per
+        # level, cloud = level slot * B + batch, each cloud its own box, so a
+        # query at level s sees exactly its level's 27 nodes.  One build, one
+        # candidate count and (below) one attached query serve them all.
+        small = [s for s in range(max_shift, -1, -1) if box_cells(s) <= _SYNC_CELLS]
+        Lc = len(small)
+        if Lc:
+            comb = self._level_grid(-1)
+            shifts = torch.tensor(small, dtype=torch.int64, device=dev)
+            g_rep = (g[None] >> shifts[:, None, None]).reshape(-1, 3)
+            b_rep = (batch[None] + B * torch.arange(Lc, device=dev)[:, None]).reshape(-1)
+            comb.reserve(n * Lc, sum(box_cells(s) for s in small) + 64, B * Lc, n)
+            comb.build_cells(g_rep, b_rep, pts.repeat(Lc, 1), num_clouds=B * Lc,
+                             num_down=0, no_sync=True)
+            c_rep = comb.point_candidates(1).view(Lc, n)
+            for i, s in enumerate(small):
+                counts[s] = c_rep[i]
+                built[s] = comb
         for s in range(max_shift, -1, -1):
-            gr = self._level_grid(s)
-            cap = box_cells(s) + 64
-            no_sync = cap <= _RESERVE_CAP
-            if no_sync:  # sized from the host-side span: no read-back
-                gr.reserve(n, cap, B, n)
-            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
-            c = gr.point_candidates(1)
-            counts[s], built[s] = c, gr
+            if s not in counts:
+                gr = self._level_grid(s)
+                cap = box_cells(s) + 64
+                no_sync = cap <= _RESERVE_CAP
+                if no_sync:  # sized from the host-side span: no read-back
+                    gr.reserve(n, cap, B, n)
+                gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
+                counts[s], built[s] = gr.point_candidates(1), gr
+            c = counts[s]
             if active is None:
                 active = c > hi_f
             else:
                 use = active & (c >= lo_f)
                 sel = torch.where(use, torch.full_like(sel, s), sel)
                 active = use & (c > hi_f)
-            # a finer level no point can select changes nothing; read the
-            # mask back only before a box large enough to matter
-            if s > 0 and box_cells(s - 1) > _SYNC_CELLS and not bool(active.any()):
+            # a finer level no point can select changes nothing; read the mask
+            # back only before a separate (large) box
+            if s > 0 and (s - 1) not in counts and not bool(active.any()):
                 break
         torch.cuda.synchronize(dev)
         t_build = (time.perf_counter() - t0) * 1e3
@@ -263,7 +282,7 @@ class AdaptiveNeighborhoodFlashKNN:
         # points ordered by selected level: each level's queries are one
         # contiguous slice of the reordered inputs and outputs
         perm = torch.argsort(sel, stable=True)
-        pts_p, g_p, batch_p = pts[perm], g[perm], batch[perm]
+        pts_p, g_p, batch_p, sel_p = pts[perm], g[perm], batch[perm], sel_l[perm]
         perm32 = perm.to(torch.int32)
         # per-level statistics of the selected level's candidate count from one
         # sort by (level, count): run bounds by searchsorted, sums by a prefix
@@ -285,39 +304,51 @@ class AdaptiveNeighborhoodFlashKNN:
         per_level = [int(v) for v in host[0].tolist()]
         out_i_p = torch.empty((n, K), dtype=torch.int64, device=dev)
         out_d_p = torch.empty((n, K), dtype=torch.float32, device=dev)
+        qkw = dict(mode=F.FK_MODE_CELLS, radius_cells=1, cut_r2=float(cut_radius) ** 2,
+                   pad_mode=F.FK_PAD_PER_QUERY, attached=True)
         start, used, stats = 0, [], []
+        s_small = small[-1] if Lc else L  # levels >= s_small are in the combined grid
         for s in range(L):
             m = per_level[s]
-            if m == 0:
+            if m:
+                used.append((s, m))
+                stats.append((float(host[1][s]) / m, int(host[2][s]), int(host[3][s])))
+            if m == 0 or s >= s_small:
                 continue
             sl = slice(start, start + m)
             start += m
             gr = built[s]
             gr.attach_queries(pts_p[sl], qcell=g_p[sl], qbatch=batch_p[sl],
                               qpad=perm32[sl], no_sync=True)
-            gr.query(K, mode=F.FK_MODE_CELLS, radius_cells=1, cut_r2=float(cut_radius) ** 2,
-                     pad_mode=F.FK_PAD_PER_QUERY, attached=True,
-                     out_idx=out_i_p[sl], out_dist=out_d_p[sl])
-            used.append((s, m))
-            stats.append((float(host[1][s]) / m, int(host[2][s]), int(host[3][s])))
+            gr.query(K, out_idx=out_i_p[sl], out_dist=out_d_p[sl], **qkw)
+        if start < n:  # the combined coarse levels: one attach, one launch
+            sl = slice(start, n)
+            sq = sel_p[sl]
+            comb = built[s_small]
+            comb.attach_queries(pts_p[sl], qcell=g_p[sl] >> sq[:, None],
+                                qbatch=batch_p[sl] + B * (max_shift - sq),
+                                qpad=perm32[sl], no_sync=True)
+            comb.query(K, out_idx=out_i_p[sl], out_dist=out_d_p[sl], **qkw)
+            # support ids of level slot i are offset by i * n; pads are query ids < n
+            out_i_p[sl] = out_i_p[sl] % n
         out_i = torch.empty_like(out_i_p)
         out_d = torch.empty_like(out_d_p)
         out_i[perm] = out_i_p
         out_d[perm] = out_d_p
-        for gr in built.values():  # device errors of the no-sync builds and queries
-            gr.status()
+        for gr in {id(v): v for v in built.values()}.values():  # device errors of
+            gr.status()                                         # the no-sync work
         t_query = (time.perf_counter() - t0) * 1e3
         self.last_stats = {
             "construction_ms": t_build, "selection_ms": 0.0,
             "query_ms": t_query, "total_ms": t_build + t_query,
             "k": K, "min_candidates_factor": self.min_candidates_factor,
             "max_candidates_factor": self.max_candidates_factor,
-            "octree_levels": max_shift + 1, "levels_built": len(built),
+            "octree_levels": max_shift + 1, "levels_built": len(counts),
             "selected_levels": [
                 {"shift": int(s), "query_points": int(m), "candidate_mean": st[0],
                  "candidate_min": int(st[1]), "candidate_max": int(st[2])}
                 for (s, m), st in zip(used, stats)],
-            "query_kernel_launches": len(used),
+            "query_kernel_launches": sum(1 for s, _ in used if s < s_small) + int(start < n),
         }
         if return_distances:
             return out_i, out_d
