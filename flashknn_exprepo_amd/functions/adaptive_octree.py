@@ -27,6 +27,7 @@ import time
 from dataclasses import dataclass, field
 from typing import Any
 
+import numpy as np
 import torch
 
 from .. import _lib as F
@@ -214,8 +215,9 @@ class AdaptiveNeighborhoodFlashKNN:
         B = lo[1] + 1
         mins = batch_min(grid_coord, batch_idx, B)
         g = (grid_coord - mins[batch_idx]).contiguous()
-        span = g.amax(0).cpu().tolist()
-        gmax = max(span)
+        # per-batch spans (one host read): they size every level's boxes
+        spans = (-batch_min(-g, batch_idx, B)).cpu().numpy()
+        gmax = int(spans.max())
         if gmax >= _MAX_COORD:
             raise ValueError("each batch's coordinate span must be smaller than 65536")
         if max_shift is None:
@@ -232,9 +234,11 @@ class AdaptiveNeighborhoodFlashKNN:
         active = None
 
         def box_cells(s):
-            # cells of all per-batch boxes at level s: at most B boxes of
-            # (span >> s) + 1 cells per axis (shifted coordinates start at 0)
-            return B * math.prod((int(v) >> s) + 1 for v in span)
+            # cells of all per-batch boxes at level s: (span >> s) + 1 cells
+            # per axis (shifted coordinates start at 0; a batch index without
+            # points has span -inf after the minimum, and no box)
+            ok = spans[:, 0] >= 0
+            return int(np.prod((spans[ok] >> s) + 1, axis=1).sum())
 
         # The coarse levels (small boxes) form one grid: the points once per
         # level, cloud = level slot * B + batch, each cloud its own box, so a
@@ -254,15 +258,19 @@ class AdaptiveNeighborhoodFlashKNN:
             for i, s in enumerate(small):
                 counts[s] = c_rep[i]
                 built[s] = comb
+        reserved = {}  # level -> its grid was reserved (no-sync build and attach)
+        if Lc:
+            reserved[-1] = True
         for s in range(max_shift, -1, -1):
             if s not in counts:
                 gr = self._level_grid(s)
                 cap = box_cells(s) + 64
                 no_sync = cap <= _RESERVE_CAP
-                if no_sync:  # sized from the host-side span: no read-back
+                if no_sync:  # sized from the host-side spans: no read-back
                     gr.reserve(n, cap, B, n)
                 gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
                 counts[s], built[s] = gr.point_candidates(1), gr
+                reserved[s] = no_sync
             c = counts[s]
             if active is None:
                 active = c > hi_f
@@ -319,7 +327,7 @@ class AdaptiveNeighborhoodFlashKNN:
             start += m
             gr = built[s]
             gr.attach_queries(pts_p[sl], qcell=g_p[sl], qbatch=batch_p[sl],
-                              qpad=perm32[sl], no_sync=True)
+                              qpad=perm32[sl], no_sync=reserved[s])
             gr.query(K, out_idx=out_i_p[sl], out_dist=out_d_p[sl], **qkw)
         if start < n:  # the combined coarse levels: one attach, one launch
             sl = slice(start, n)
