@@ -38,6 +38,7 @@ FK_DIST_SQRT = 2
 FK_QUERIES_SELF = 0
 FK_QUERIES_ATTACHED = 1
 FK_BUILD_NO_SYNC = 1
+FK_BUILD_COLUMNS = 2
 FK_QUERY_PRUNE = 1
 FK_QUERY_ONE_LANE = 2
 

@@ -5,6 +5,7 @@
 // synchronisation-free builds).  Every entry point validates its arguments,
 // enqueues kernels on the caller's stream and returns an fk_status; a
 // message for the last failure is kept per thread (fk_last_error).
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -47,7 +48,10 @@ int ok() {
     if (c_ != FK_OK) return c_; \
   } while (0)
 
-constexpr int64_t kHardCellCap = 1ll << 28;  // dense cells per build (sync mode)
+// Dense cells per build (sync mode). Linear cell ids are int32 and each box is
+// below 2^30 cells, so a total of 2^30 keeps every id, and every scan entry
+// (cells + 3), below INT32_MAX; at the cap the four cell tables take 16 GiB.
+constexpr int64_t kHardCellCap = 1ll << 30;
 
 inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
@@ -90,6 +94,11 @@ struct fk_grid {
   float4 *srec = nullptr;
   int4 *scell = nullptr;
   int64_t cap_scell = 0;
+  // column layout: radix sort buffers
+  uint64_t *ckeys0 = nullptr, *ckeys1 = nullptr;
+  int32_t *cperm0 = nullptr, *cperm1 = nullptr;
+  uint8_t *sort_tmp = nullptr;
+  int64_t cap_ck0 = 0, cap_ck1 = 0, cap_cp0 = 0, cap_cp1 = 0, cap_tmp = 0;
   // per cell
   int64_t cap_cells = 0, cap_c2 = 0;
   int32_t *counts = nullptr, *cell_start = nullptr;
@@ -108,6 +117,7 @@ struct fk_grid {
   uint64_t *pend_keys = nullptr;
   // host state of the last build
   int built = 0, from_xyz = 0, num_down = 0, reserved = 0;
+  int columns = 0;  // last build used the column layout
   // query counter sets: the set of the last plain first pass, which sets are
   // known to be zero, and the set fk_query_stats reports
   int qset = 0, stat_set = 0;
@@ -116,6 +126,9 @@ struct fk_grid {
   int64_t stat_nq = 0, batched_nq = 0;
   bool qclean[2] = {true, true};
   int64_t n_points = 0, n_cells = -1, n_queries = -1;
+  // upper bound of the last build's table entries (its exact count after a
+  // synchronising build, the capacity it was held to after a no-sync one)
+  int64_t build_cap = 0;
   int32_t n_clouds = 0;
   // captured batched step
   hipGraph_t graph = nullptr;
@@ -204,6 +217,12 @@ BuildBuffers build_buffers(fk_grid *g) {
   b.srec = g->srec;
   b.scell = g->scell;
   b.cap_cells = g->cap_cells - 1;
+  b.ckeys[0] = g->ckeys0;
+  b.ckeys[1] = g->ckeys1;
+  b.cperm[0] = g->cperm0;
+  b.cperm[1] = g->cperm1;
+  b.sort_tmp = g->sort_tmp;
+  b.sort_tmp_bytes = (size_t)g->cap_tmp;
   return b;
 }
 
@@ -244,21 +263,59 @@ int finish_build(fk_grid *g, const float *xyz, const int64_t *grid, int64_t n,
                  int32_t flags, hipStream_t s) {
   const bool nosync = flags & FK_BUILD_NO_SYNC;
   BuildBuffers bb = build_buffers(g);
-  if (!nosync) bb.cap_cells = kHardCellCap;
-  FK_HIP(launch_layout(bb, B, offsets, inv_cell, s));
+  if (nosync) {
+    // A no-sync build stays within the reserved query tables too, so that a
+    // no-sync attach always fits (a synchronising build may have grown the
+    // cell tables past the reserve).
+    if (g->qcounts && g->cap_qcells - 1 < bb.cap_cells) bb.cap_cells = g->cap_qcells - 1;
+  } else {
+    bb.cap_cells = kHardCellCap;
+  }
+  g->columns = (flags & FK_BUILD_COLUMNS) ? 1 : 0;
+  FK_HIP(launch_layout(bb, B, offsets, inv_cell, g->columns, s));
   int64_t hint;
   if (nosync) {
-    hint = g->cap_cells - 1;
+    hint = bb.cap_cells;
     g->n_cells = -1;
   } else {
     FK_CHECK(read_scalars(g, s));
+    if (g->h_pinned[kStatus] == FK_ERR_GRID_TOO_LARGE && !g->columns) {
+      // Too many cells for one table entry each: one entry per (x, y)
+      // column, points sorted by z inside the column.
+      FK_HIP(launch_fill_i32(g->scalars + kStatus, 0, 1, s));
+      FK_HIP(launch_layout(bb, B, offsets, inv_cell, 1, s));
+      FK_CHECK(read_scalars(g, s));
+      g->columns = 1;
+    }
     if (g->h_pinned[kStatus] != 0) return status_message(g->h_pinned[kStatus]);
     hint = g->h_pinned[kTotalCells];
     FK_CHECK(ensure_cells(g, hint));
     g->n_cells = hint;
     bb = build_buffers(g);
   }
-  FK_HIP(launch_sort(bb, xyz, grid, n, hint, s));
+  if (g->columns) {
+    auto bits = [](uint64_t v) {  // bits to hold 0..v
+      int b = 0;
+      while (v >> b) ++b;
+      return b;
+    };
+    const int depth = g->h_pinned[kMaxDepth];
+    const int depth_bits = bits(depth > 0 ? (uint64_t)depth - 1 : 0);
+    const int key_bits = std::max(1, depth_bits + bits((uint64_t)hint));
+    size_t tmp = 0;
+    FK_HIP(column_sort_tmp_bytes(n, key_bits, &tmp));
+    const int64_t m = n < 1 ? 1 : n;
+    FK_CHECK(grow(g->ckeys0, g->cap_ck0, m, "column keys"));
+    FK_CHECK(grow(g->ckeys1, g->cap_ck1, m, "column keys"));
+    FK_CHECK(grow(g->cperm0, g->cap_cp0, m, "column order"));
+    FK_CHECK(grow(g->cperm1, g->cap_cp1, m, "column order"));
+    FK_CHECK(grow(g->sort_tmp, g->cap_tmp, (int64_t)tmp + 1, "column sort"));
+    bb = build_buffers(g);
+    FK_HIP(launch_sort_columns(bb, xyz, grid, n, hint, depth_bits, key_bits, s));
+  } else {
+    FK_HIP(launch_sort(bb, xyz, grid, n, hint, s));
+  }
+  g->build_cap = hint;
   g->built = 1;
   g->n_points = n;
   g->n_clouds = B;
@@ -331,7 +388,8 @@ int fk_grid_destroy(fk_grid *g) {
                   g->pt_lin, g->pt_rank, g->srec, g->scell, g->counts,
                   g->cell_start, g->qcounts, g->qstart, g->scan_part, g->qrec,
                   g->qcell, g->q_cell_abs, g->qpad, g->q_lin, g->q_rank,
-                  g->pend_q, g->pend_keys};
+                  g->pend_q, g->pend_keys, g->ckeys0, g->ckeys1, g->cperm0,
+                  g->cperm1, g->sort_tmp};
   for (void *p : bufs)
     if (p) note(hipFree(p));
   if (g->h_pinned) note(hipHostFree(g->h_pinned));
@@ -369,6 +427,8 @@ int fk_grid_build_cells(fk_grid *g, const float *xyz, const int64_t *grid_coord,
                 "num_down in [0, 30] and grid_coord required");
   hipStream_t s = S(stream);
   const bool nosync = flags & FK_BUILD_NO_SYNC;
+  if (nosync && (flags & FK_BUILD_COLUMNS))
+    return fail(FK_ERR_INVALID_ARG, "FK_BUILD_COLUMNS needs a synchronising build");
   if (nosync) {
     if (n > g->cap_points || num_clouds > g->cap_clouds || !g->counts)
       return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs fk_grid_reserve "
@@ -399,6 +459,8 @@ int fk_grid_build_xyz(fk_grid *g, const float *xyz, const int32_t *offsets,
                 "offsets and inv_cell required");
   hipStream_t s = S(stream);
   const bool nosync = flags & FK_BUILD_NO_SYNC;
+  if (nosync && (flags & FK_BUILD_COLUMNS))
+    return fail(FK_ERR_INVALID_ARG, "FK_BUILD_COLUMNS needs a synchronising build");
   if (nosync) {
     if (n > g->cap_points || num_clouds > g->cap_clouds || !g->counts)
       return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs fk_grid_reserve "
@@ -437,9 +499,11 @@ int fk_grid_attach_queries(fk_grid *g, const float *qxyz, const int64_t *qcell,
   const bool nosync = flags & FK_BUILD_NO_SYNC;
   int64_t hint = g->n_cells;
   if (nosync || hint < 0) {
-    if (nq > g->cap_q || !g->qcounts || g->cap_qcells < g->cap_cells)
+    // the query tables hold the build's table entries + 1 (one bucket for
+    // queries of clouds without cells)
+    if (nq > g->cap_q || !g->qcounts || g->build_cap > g->cap_qcells - 1)
       return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs reserved query capacity");
-    hint = g->cap_cells - 1;
+    hint = g->build_cap;
   } else {
     FK_CHECK(ensure_queries(g, nq));
     FK_CHECK(ensure_qcells(g, hint));
@@ -528,7 +592,9 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
     return fail(FK_ERR_INVALID_ARG, "fk_query: cut_r2 is NaN");
   memset(&a, 0, sizeof(a));
   a.srec = g->srec;
+  a.scell = g->scell;
   a.cell_start = g->cell_start;
+  a.columns = g->columns;
   a.clouds = g->clouds;
   a.qrec = self ? g->srec : g->qrec;
   a.qcell = self ? g->scell : g->qcell;

@@ -339,14 +339,19 @@ __global__ void k_auto_cell(const int32_t *bbox, const int32_t *offsets,
   inv_out[c] = (float)(1.0 / (double)s);
 }
 
-// One block: per-cloud box sizes, exclusive scan of cell counts, CloudInfo.
+// One block: per-cloud box sizes, exclusive scan of table entries (cells,
+// or columns in the column layout), CloudInfo.
 __global__ void __launch_bounds__(1024)
 k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
-         const float *inv_cell, int64_t cap_cells, CloudInfo *clouds,
-         int32_t *scalars) {
+         const float *inv_cell, int64_t cap_cells, int32_t columns,
+         CloudInfo *clouds, int32_t *scalars) {
   __shared__ int64_t warp_sums[16];
   __shared__ int64_t carry;
-  if (threadIdx.x == 0) carry = 0;
+  __shared__ int max_depth;
+  if (threadIdx.x == 0) {
+    carry = 0;
+    max_depth = 0;
+  }
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int c0 = 0; c0 < B; c0 += 1024) {
@@ -359,11 +364,14 @@ k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
         mnx = b[0]; mny = b[1]; mnz = b[2];
         const int64_t ex = (int64_t)b[3] - b[0] + 1, ey = (int64_t)b[4] - b[1] + 1,
                       ez = (int64_t)b[5] - b[2] + 1;
-        if (ex * ey * ez > kCellLimit) {
+        // ez < 2^31 (|cell| < 2^30), so nz fits an int in both layouts
+        const int64_t entries = columns ? ex * ey : ex * ey * ez;
+        if (entries > kCellLimit) {
           raise_status(scalars, FK_ERR_GRID_TOO_LARGE);
         } else {
           nx = (int)ex; ny = (int)ey; nz = (int)ez;
-          cells = ex * ey * ez;
+          cells = entries;
+          if (columns) atomicMax(&max_depth, nz);
         }
       }
     }
@@ -387,7 +395,8 @@ k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
       ci.first_id = offsets ? (c ? offsets[c - 1] : 0) : 0;
       ci.num_points = offsets ? offsets[c] - ci.first_id : 0;
       ci.inv_cell = inv_cell ? inv_cell[c] : 0.f;
-      ci.pad0 = ci.pad1 = 0;
+      ci.columns = columns;
+      ci.pad1 = 0;
       clouds[c] = ci;
     }
     __syncthreads();
@@ -396,6 +405,7 @@ k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
   }
   if (threadIdx.x == 0) {
     const int64_t total = carry;
+    scalars[kMaxDepth] = max_depth;
     if (total > cap_cells) {
       raise_status(scalars, FK_ERR_GRID_TOO_LARGE);
       scalars[kTotalCells] = (int32_t)(cap_cells < INT_MAX ? cap_cells : INT_MAX);
@@ -403,11 +413,6 @@ k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
       scalars[kTotalCells] = (int32_t)total;
     }
   }
-}
-
-__device__ __forceinline__ int32_t lin_of(const CloudInfo &ci, int lx, int ly,
-                                          int lz) {
-  return ci.base + (lx * ci.ny + ly) * ci.nz + lz;
 }
 
 // In-cell rank from a counting atomic, aggregated over the lanes of a wave
@@ -744,15 +749,14 @@ hipError_t launch_auto_cell(int32_t *bbox, const float *xyz,
 
 hipError_t launch_layout(const BuildBuffers &b, int32_t B,
                          const int32_t *offsets, const float *inv_cell,
-                         hipStream_t s) {
+                         int32_t columns, hipStream_t s) {
   hipLaunchKernelGGL(k_layout, dim3(1), dim3(1024), 0, s, b.bbox, B, offsets,
-                     inv_cell, b.cap_cells, b.clouds, b.scalars);
+                     inv_cell, b.cap_cells, columns, b.clouds, b.scalars);
   return hipGetLastError();
 }
 
-hipError_t launch_sort(const BuildBuffers &b, const float *xyz,
-                       const int64_t *grid, int64_t n, int64_t cells_hint,
-                       hipStream_t s) {
+hipError_t launch_count_scan(const BuildBuffers &b, int64_t n,
+                             int64_t cells_hint, hipStream_t s) {
   unsigned nfill = blocks_for(cells_hint + 1);
   if (nfill > 8192) nfill = 8192;
   hipLaunchKernelGGL(k_fill_dev, dim3(nfill), dim3(kBlock), 0, s, b.counts, 0,
@@ -761,8 +765,14 @@ hipError_t launch_sort(const BuildBuffers &b, const float *xyz,
     hipLaunchKernelGGL(k_count, dim3(blocks_for(n)), dim3(kBlock), 0, s,
                        b.pt_cell, n, b.clouds, b.counts, b.pt_lin, b.pt_rank,
                        b.scalars);
-  hipError_t e = launch_scan(b.counts, b.cell_start, b.scalars + kTotalCells, 0,
-                             cells_hint, b.scan_part, s);
+  return launch_scan(b.counts, b.cell_start, b.scalars + kTotalCells, 0,
+                     cells_hint, b.scan_part, s);
+}
+
+hipError_t launch_sort(const BuildBuffers &b, const float *xyz,
+                       const int64_t *grid, int64_t n, int64_t cells_hint,
+                       hipStream_t s) {
+  hipError_t e = launch_count_scan(b, n, cells_hint, s);
   if (e != hipSuccess) return e;
   if (n > 0)
     hipLaunchKernelGGL(k_scatter, dim3(blocks_for(n)), dim3(kBlock), 0, s, xyz,

@@ -17,6 +17,7 @@ constexpr int kMaxLaneK = 64;
 enum Scalar : int {
   kStatus = 0,        // first error raised by a kernel (fk_status), 0 = ok
   kTotalCells = 1,    // cells of all per-cloud boxes
+  kMaxDepth = 2,      // column layouts: largest box depth nz of a cloud
   kQueueHead = 3,     // batched mode: next work item
   kCounters = 16,     // query counter sets: kCounters + kCntStride * set
   kNumScalars = 32
@@ -34,20 +35,36 @@ enum Counter : int {
 };
 constexpr int kBatchedSet = 2;
 
-// Dense cell box of one cloud.  Cell (lx,ly,lz) of cloud c (local
-// coordinates, 0 <= l < n) has linear id base + (lx*ny + ly)*nz + lz, so the
-// three cells of a column segment (x, y, z-1..z+1) are adjacent and their
-// points form one contiguous run of the cell-sorted arrays.
+// Cell box of one cloud.  Dense layout: cell (lx,ly,lz) (local coordinates,
+// 0 <= l < n) has table entry base + (lx*ny + ly)*nz + lz, so the three cells
+// of a column segment (x, y, z-1..z+1) are adjacent and their points form one
+// contiguous run of the cell-sorted arrays.  Column layout (boxes too large
+// for one entry per cell): column (lx, ly) has entry base + lx*ny + ly, the
+// points of a column are sorted by lz, and a segment's run is found by
+// binary search over the sorted local z of the column.
 struct CloudInfo {
-  int32_t base;            // first linear cell id of the box
+  int32_t base;            // first table entry of the box
   int32_t nx, ny, nz;      // box size in cells (0 => empty cloud)
   int32_t minx, miny, minz;// absolute cell coordinate of local (0,0,0)
   int32_t first_id;        // offsets-built grids: first point id of the cloud
   float inv_cell;          // xyz-built grids: 1 / cell size (0 otherwise)
   int32_t num_points;      // points of the cloud
-  int32_t pad0, pad1;
+  int32_t columns;         // 1: column layout
+  int32_t pad1;
 };
 static_assert(sizeof(CloudInfo) == 48, "CloudInfo layout");
+
+// Table entries of a cloud's box.
+__host__ __device__ __forceinline__ int32_t table_entries(const CloudInfo &ci) {
+  return ci.columns ? ci.nx * ci.ny : ci.nx * ci.ny * ci.nz;
+}
+
+// Table entry of local cell (lx, ly, lz) inside the box.
+__device__ __forceinline__ int32_t lin_of(const CloudInfo &ci, int lx, int ly,
+                                          int lz) {
+  return ci.columns ? ci.base + lx * ci.ny + ly
+                    : ci.base + (lx * ci.ny + ly) * ci.nz + lz;
+}
 
 __device__ __forceinline__ void raise_status(int32_t *scalars, int code) {
   atomicCAS(scalars + kStatus, 0, code);
@@ -84,6 +101,11 @@ struct BuildBuffers {
   float4 *srec;          // [n]   sorted (x, y, z, id bits)
   int4 *scell;           // [n]   sorted local cell + cloud
   int64_t cap_cells;
+  // column layout only: radix sort of (table entry, lz) keys
+  uint64_t *ckeys[2];    // [n] each
+  int32_t *cperm[2];     // [n] each
+  void *sort_tmp;
+  size_t sort_tmp_bytes;
 };
 
 hipError_t launch_build_cells(const BuildBuffers &b, const float *xyz,
@@ -98,13 +120,27 @@ hipError_t launch_auto_cell(int32_t *bbox, const float *xyz,
                             const int32_t *offsets, int32_t B, int64_t n,
                             const int32_t *k_per_cloud, int32_t k, float scale,
                             float *side_out, float *inv_out, hipStream_t s);
+// columns: 0 dense boxes, 1 column layout (see CloudInfo)
 hipError_t launch_layout(const BuildBuffers &b, int32_t num_clouds,
                          const int32_t *offsets, const float *inv_cell,
-                         hipStream_t s);
+                         int32_t columns, hipStream_t s);
 // count / scan / scatter; cells_hint = host upper bound of total cells
 hipError_t launch_sort(const BuildBuffers &b, const float *xyz,
                        const int64_t *grid, int64_t n, int64_t cells_hint,
                        hipStream_t s);
+// Column layout: counts and start table as launch_sort, then the points
+// ordered by (table entry, lz) with a radix sort of key_bits-bit keys
+// (entry << depth_bits | lz).  (fk_columns.hip)
+hipError_t launch_sort_columns(const BuildBuffers &b, const float *xyz,
+                               const int64_t *grid, int64_t n,
+                               int64_t cells_hint, int depth_bits, int key_bits,
+                               hipStream_t s);
+// Temporary bytes of the radix sort of n pairs with key_bits-bit keys.
+hipError_t column_sort_tmp_bytes(int64_t n, int key_bits, size_t *bytes);
+// Counting-sort steps shared by both layouts (fk_build.hip): counts per
+// table entry, in-entry ranks, start table.
+hipError_t launch_count_scan(const BuildBuffers &b, int64_t n,
+                             int64_t cells_hint, hipStream_t s);
 
 struct QueryBuffers {
   float4 *qrec;          // [nq] sorted (x, y, z, output row bits)
@@ -126,6 +162,7 @@ hipError_t launch_attach(const BuildBuffers &b, const QueryBuffers &q,
 
 struct QueryArgs {
   const float4 *srec;
+  const int4 *scell;     // sorted local cells (column layouts search their z)
   const int32_t *cell_start;
   const CloudInfo *clouds;
   const float4 *qrec;
@@ -145,6 +182,7 @@ struct QueryArgs {
   int32_t passes;        // bit 0 reset, bit 1 first pass, bit 2 expansion
   int32_t prune;         // FK_QUERY_PRUNE: prune neighbour cells mid-scan
   int32_t one_lane;      // FK_QUERY_ONE_LANE: one lane per query (64 per wave)
+  int32_t columns;       // the grid uses the column layout (see CloudInfo)
   int32_t *scalars;
   int32_t *qcnt;         // this call's counter set (Counter slots)
   int32_t *qcnt_next;    // set the first pass clears for the next call (or null)
@@ -162,6 +200,25 @@ hipError_t launch_query(const QueryArgs &a, hipStream_t s);
 hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s);
 
 int kclass_of(int k);  // 8, 16, 32, 64 or 128
+// The query launches built for the column layout (fk_query.hip compiled with
+// FK_COLUMNS=1); the launches above forward to them when a.columns is set.
+hipError_t launch_query_columns(const QueryArgs &a, hipStream_t s);
+hipError_t launch_batched_columns(const QueryArgs &a, int32_t max_k,
+                                  hipStream_t s);
+hipError_t launch_candidate_count_columns(const QueryArgs &a,
+                                          unsigned long long *total,
+                                          hipStream_t s);
+hipError_t launch_point_candidates_columns(const QueryArgs &a, int32_t *out,
+                                           hipStream_t s);
+hipError_t launch_cloud_candidates_columns(const QueryArgs &a,
+                                           unsigned long long *acc,
+                                           hipStream_t s);
+hipError_t launch_adapt_cell_columns(const QueryArgs &a, unsigned long long *acc,
+                                     const int32_t *k_per_cloud, int32_t k,
+                                     float target_ratio,
+                                     const float *ratio_per_cloud,
+                                     float exponent, float *side_io,
+                                     float *inv_io, hipStream_t s);
 hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
                                   hipStream_t s);
 hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,

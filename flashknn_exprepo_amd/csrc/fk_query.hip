@@ -31,10 +31,26 @@
 // go to a pending list and a second kernel grows their block shell by shell
 // (radius from the current K-th distance, doubled while it is infinite)
 // until they certify.
+//
+// The file is compiled twice: with FK_COLUMNS=0 for grids with a dense cell
+// table, and with FK_COLUMNS=1 for the column layout (column_range searches
+// z inside a column).  Only the lookup differs; keeping it out of the dense
+// kernels keeps their register allocation.  The public launchers of the
+// column build carry a _columns suffix, and the dense ones forward to them
+// when the grid uses the column layout.
 #include "fk_internal.h"
 
 #include <cfloat>
 #include <climits>
+
+#ifndef FK_COLUMNS
+#define FK_COLUMNS 0
+#endif
+#if FK_COLUMNS
+#define FK_LAYOUT(name) name##_columns
+#else
+#define FK_LAYOUT(name) name
+#endif
 
 namespace fk {
 
@@ -369,8 +385,12 @@ __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, int sub,
 }
 
 // Column segment (cx, cy, z0..z1) of the query's cloud box -> [b, e).
+// Dense boxes read the two ends from the cell table; a column layout reads
+// the column's run and narrows it to z0..z1 by binary search over the
+// sorted local z of its records.
 __device__ __forceinline__ void column_range(const CloudInfo &ci,
                                              const int32_t *__restrict__ cs,
+                                             const int4 *__restrict__ sc,
                                              int cx, int cy, int z0, int z1,
                                              int &b, int &e) {
   b = e = 0;
@@ -378,9 +398,28 @@ __device__ __forceinline__ void column_range(const CloudInfo &ci,
   z0 = z0 < 0 ? 0 : z0;
   z1 = z1 >= ci.nz ? ci.nz - 1 : z1;
   if (z0 > z1) return;
-  const int col = ci.base + (cx * ci.ny + cy) * ci.nz;
-  b = cs[col + z0];
-  e = cs[col + z1 + 1];
+  if constexpr (!FK_COLUMNS) {
+    (void)sc;
+    const int col = ci.base + (cx * ci.ny + cy) * ci.nz;
+    b = cs[col + z0];
+    e = cs[col + z1 + 1];
+  } else {
+    const int col = ci.base + cx * ci.ny + cy;
+    int lo = cs[col];
+    const int hi = cs[col + 1];
+    int h = hi;
+    while (lo < h) {  // first record with z >= z0
+      const int m = lo + ((h - lo) >> 1);
+      if (sc[m].z < z0) lo = m + 1; else h = m;
+    }
+    b = lo;
+    h = hi;
+    while (lo < h) {  // first record with z > z1
+      const int m = lo + ((h - lo) >> 1);
+      if (sc[m].z <= z1) lo = m + 1; else h = m;
+    }
+    e = lo;
+  }
 }
 
 // Scan the full block of half-width R around the query's cell.  The
@@ -429,6 +468,7 @@ struct Cull {
 // the threshold can still reach: [b, e) (empty when the column is pruned).
 __device__ __forceinline__ void culled_column(const CloudInfo &ci,
                                               const int32_t *__restrict__ cs,
+                                              const int4 *__restrict__ sc,
                                               const Cull &cu, int4 qc, int dx,
                                               int dy, int &b, int &e) {
   b = e = 0;
@@ -436,7 +476,7 @@ __device__ __forceinline__ void culled_column(const CloudInfo &ci,
   if (!cu.keep(xy)) return;
   const int z0 = cu.keep(xy + cu.gap2(2, -1)) ? qc.z - 1 : qc.z;
   const int z1 = cu.keep(xy + cu.gap2(2, 1)) ? qc.z + 1 : qc.z;
-  column_range(ci, cs, qc.x + dx, qc.y + dy, z0, z1, b, e);
+  column_range(ci, cs, sc, qc.x + dx, qc.y + dy, z0, z1, b, e);
 }
 
 // The candidate ranges of one query (its own column, then the 8 around it)
@@ -577,7 +617,7 @@ __device__ __forceinline__ void ring_columns(const QueryArgs &a, const CloudInfo
   for (int c = 0; c < 8; ++c) {
     const int j = c < 4 ? c : c + 1;  // 3x3 raster order without the centre
     b[o + c] = e[o + c] = 0;
-    if (active) culled_column(ci, a.cell_start, cu, qc, j / 3 - 1, j % 3 - 1, b[o + c], e[o + c]);
+    if (active) culled_column(ci, a.cell_start, a.scell, cu, qc, j / 3 - 1, j % 3 - 1, b[o + c], e[o + c]);
   }
 }
 
@@ -597,7 +637,7 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
     if (R == 1) {
       int b0 = 0, e0 = 0;
       if (active)
-        column_range(ci, a.cell_start, qc.x, qc.y, qc.z - 1, qc.z + 1, b0, e0);
+        column_range(ci, a.cell_start, a.scell, qc.x, qc.y, qc.z - 1, qc.z + 1, b0, e0);
       if (a.prune) {  // launch-uniform
         {
           const int b[1] = {b0}, e[1] = {e0};
@@ -631,7 +671,7 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
   {
     int b = 0, e = 0;
     if (active)
-      column_range(ci, a.cell_start, qc.x, qc.y, qc.z - R, qc.z + R, b, e);
+      column_range(ci, a.cell_start, a.scell, qc.x, qc.y, qc.z - R, qc.z + R, b, e);
     scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
   }
   for (int dx = -R; dx <= R; ++dx) {
@@ -639,7 +679,7 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
       if (dx == 0 && dy == 0) continue;
       int b = 0, e = 0;
       if (active)
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - R,
+        column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, qc.z - R,
                      qc.z + R, b, e);
       scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
     }
@@ -1001,10 +1041,10 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
       if (sj < ncol) {
         const int dx = x0 + (int)(sj / wy), dy = y0 + (int)(sj % wy);
         const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
-        column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zlo, inner ? zin_lo : zhi,
+        column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, zlo, inner ? zin_lo : zhi,
                      b[2 * j], en[2 * j]);
         if (inner)
-          column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, zin_hi, zhi,
+          column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, zin_hi, zhi,
                        b[2 * j + 1], en[2 * j + 1]);
       }
     }
@@ -1223,7 +1263,7 @@ k_batched(QueryArgs a) {
     const int c = item_to_cloud(prefix, a.num_clouds, item);
     const CloudInfo ci = a.clouds[c];
     const int qb = a.cell_start[ci.base];
-    const int qe = a.cell_start[ci.base + ci.nx * ci.ny * ci.nz];
+    const int qe = a.cell_start[ci.base + table_entries(ci)];
     const int q0 = qb + (item - prefix[c]) * (kWave / G);
     const int k = a.k_per_cloud[c];
     if (k < 1 || k > a.k) {
@@ -1242,6 +1282,7 @@ k_batched(QueryArgs a) {
   }
 }
 
+#if !FK_COLUMNS
 // Per-cloud tile counts -> exclusive prefix (one block).
 __global__ void __launch_bounds__(1024)
 k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
@@ -1256,7 +1297,7 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
     int tiles = 0;
     if (c < B) {
       const CloudInfo ci = clouds[c];
-      const int n = cell_start[ci.base + ci.nx * ci.ny * ci.nz] - cell_start[ci.base];
+      const int n = cell_start[ci.base + table_entries(ci)] - cell_start[ci.base];
       tiles = (n + tile - 1) / tile;
     }
     int v = tiles;
@@ -1281,6 +1322,7 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
     qcnt[kCntExpanded] = 0;
   }
 }
+#endif
 
 // Sum over queries of the candidate count |C(q)| of the block of
 // half-width R (the reference's traverse statistic,
@@ -1298,7 +1340,7 @@ __device__ __forceinline__ unsigned long long query_candidates(const QueryArgs &
   for (int dx = -a.R; dx <= a.R; ++dx)
     for (int dy = -a.R; dy <= a.R; ++dy) {
       int b, e;
-      column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
+      column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, qc.z - a.R,
                    qc.z + a.R, b, e);
       c += (unsigned long long)(e - b);
     }
@@ -1333,7 +1375,7 @@ __global__ void k_point_candidates(const QueryArgs a, int32_t *out) {
   for (int dx = -a.R; dx <= a.R; ++dx)
     for (int dy = -a.R; dy <= a.R; ++dy) {
       int b, e;
-      column_range(ci, a.cell_start, qc.x + dx, qc.y + dy, qc.z - a.R,
+      column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, qc.z - a.R,
                    qc.z + a.R, b, e);
       c += e - b;
     }
@@ -1506,11 +1548,18 @@ hipError_t launch_batched_t(const QueryArgs &a, hipStream_t s) {
 
 }  // namespace
 
+#if !FK_COLUMNS
 int kclass_of(int k) {
   return k <= 8 ? 8 : (k <= 16 ? 16 : (k <= 32 ? 32 : (k <= 64 ? 64 : 128)));
 }
+#define FK_FORWARD(call) \
+  if (a.columns) return call
+#else
+#define FK_FORWARD(call)
+#endif
 
-hipError_t launch_query(const QueryArgs &a, hipStream_t s) {
+hipError_t FK_LAYOUT(launch_query)(const QueryArgs &a, hipStream_t s) {
+  FK_FORWARD(launch_query_columns(a, s));
   if (a.passes & 1)
     hipLaunchKernelGGL(k_reset_query_counters, dim3(1), dim3(64), 0, s, a.qcnt);
   if (a.nq <= 0) return hipGetLastError();
@@ -1530,24 +1579,29 @@ hipError_t launch_query(const QueryArgs &a, hipStream_t s) {
   }
 }
 
-hipError_t launch_candidate_count(const QueryArgs &a, unsigned long long *total,
-                                  hipStream_t s) {
+hipError_t FK_LAYOUT(launch_candidate_count)(const QueryArgs &a,
+                                             unsigned long long *total,
+                                             hipStream_t s) {
+  FK_FORWARD(launch_candidate_count_columns(a, total, s));
   if (a.nq <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_candidate_count, dim3((a.nq + kCountBlock - 1) / kCountBlock),
                      dim3(kCountBlock), 0, s, a, total);
   return hipGetLastError();
 }
 
-hipError_t launch_point_candidates(const QueryArgs &a, int32_t *out,
-                                   hipStream_t s) {
+hipError_t FK_LAYOUT(launch_point_candidates)(const QueryArgs &a, int32_t *out,
+                                              hipStream_t s) {
+  FK_FORWARD(launch_point_candidates_columns(a, out, s));
   if (a.nq <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_point_candidates, dim3((a.nq + 255) / 256), dim3(256), 0,
                      s, a, out);
   return hipGetLastError();
 }
 
-hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
-                                   hipStream_t s) {
+hipError_t FK_LAYOUT(launch_cloud_candidates)(const QueryArgs &a,
+                                              unsigned long long *acc,
+                                              hipStream_t s) {
+  FK_FORWARD(launch_cloud_candidates_columns(a, acc, s));
   const int B = a.num_clouds;
   hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
   if (a.nq > 0)
@@ -1556,11 +1610,13 @@ hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
   return hipGetLastError();
 }
 
-hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *acc,
-                             const int32_t *k_per_cloud, int32_t k,
-                             float target_ratio, const float *ratio_pc,
-                             float exponent, float *side_io, float *inv_io,
-                             hipStream_t s) {
+hipError_t FK_LAYOUT(launch_adapt_cell)(const QueryArgs &a, unsigned long long *acc,
+                                        const int32_t *k_per_cloud, int32_t k,
+                                        float target_ratio, const float *ratio_pc,
+                                        float exponent, float *side_io,
+                                        float *inv_io, hipStream_t s) {
+  FK_FORWARD(launch_adapt_cell_columns(a, acc, k_per_cloud, k, target_ratio,
+                                       ratio_pc, exponent, side_io, inv_io, s));
   const int B = a.num_clouds;
   hipLaunchKernelGGL(k_zero_u64, dim3((B + 255) / 256), dim3(256), 0, s, acc, B);
   if (a.nq > 0)
@@ -1572,6 +1628,7 @@ hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *acc,
   return hipGetLastError();
 }
 
+#if !FK_COLUMNS
 int batched_tile(int32_t max_k) {
   switch (kclass_of(max_k)) {
     case 8: return kWave / batched_lanes<8>();
@@ -1589,7 +1646,11 @@ hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
   return hipGetLastError();
 }
 
-hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s) {
+#endif  // !FK_COLUMNS
+
+hipError_t FK_LAYOUT(launch_batched)(const QueryArgs &a, int32_t max_k,
+                                     hipStream_t s) {
+  FK_FORWARD(launch_batched_columns(a, max_k, s));
   switch (kclass_of(max_k)) {
     case 8: return launch_batched_t<8>(a, s);
     case 16: return launch_batched_t<16>(a, s);

@@ -19,6 +19,10 @@ def _dev(t: torch.Tensor, dtype: torch.dtype, name: str) -> torch.Tensor:
     return t.to(dtype).contiguous()
 
 
+def _build_flags(no_sync: bool, columns: bool) -> int:
+    return (F.FK_BUILD_NO_SYNC if no_sync else 0) | (F.FK_BUILD_COLUMNS if columns else 0)
+
+
 class Grid:
     """Cell-sorted support points of one or more clouds on one HIP device."""
 
@@ -61,8 +65,10 @@ class Grid:
                 "fk_grid_reserve")
 
     def build_cells(self, grid_coord, batch_idx=None, xyz=None, num_clouds=None,
-                    num_down=0, no_sync=False):
-        """cell = grid_coord >> num_down, cloud = batch_idx (FlashKNN layout)."""
+                    num_down=0, no_sync=False, columns=False):
+        """cell = grid_coord >> num_down, cloud = batch_idx (FlashKNN layout).
+        ``columns`` forces the column layout (taken by itself when the dense
+        cell boxes exceed the table capacity); results do not change."""
         g = _dev(grid_coord, torch.int64, "grid_coord")
         if g.dim() != 2 or g.shape[1] != 3:
             raise ValueError("grid_coord must be [N, 3]")
@@ -83,12 +89,12 @@ class Grid:
                 raise ValueError("xyz must be [N, 3] like grid_coord")
         F.check(self._L.fk_grid_build_cells(
             self._h, F.ptr(x), F.ptr(g), F.ptr(b), n, int(num_clouds), int(num_down),
-            F.FK_BUILD_NO_SYNC if no_sync else 0, F.stream_ptr()), "fk_grid_build_cells")
+            _build_flags(no_sync, columns), F.stream_ptr()), "fk_grid_build_cells")
         self.from_xyz = False
         self.num_points, self.num_clouds, self.num_queries = n, int(num_clouds), -1
         return self
 
-    def build_xyz(self, xyz, offsets, inv_cell, no_sync=False):
+    def build_xyz(self, xyz, offsets, inv_cell, no_sync=False, columns=False):
         """cell = floor(xyz * inv_cell[c]); clouds by cumulative end offsets."""
         x = _dev(xyz, torch.float32, "xyz")
         if x.dim() != 2 or x.shape[1] != 3:
@@ -99,7 +105,7 @@ class Grid:
             raise ValueError("one inverse cell size per cloud")
         F.check(self._L.fk_grid_build_xyz(
             self._h, F.ptr(x), F.ptr(o), int(o.shape[0]), x.shape[0], F.ptr(inv),
-            F.FK_BUILD_NO_SYNC if no_sync else 0, F.stream_ptr()), "fk_grid_build_xyz")
+            _build_flags(no_sync, columns), F.stream_ptr()), "fk_grid_build_xyz")
         self.from_xyz = True
         self.num_points, self.num_clouds, self.num_queries = x.shape[0], int(o.shape[0]), -1
         return self

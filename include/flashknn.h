@@ -10,8 +10,10 @@
  * calling thread.
  *
  * A grid handle (fk_grid) owns the device tables of one build: cell-sorted
- * support records, per-cell start offsets of a dense per-cloud cell box,
- * per-cloud layout, and optionally an attached, cell-sorted query list.
+ * support records, per-cell start offsets of a dense per-cloud cell box
+ * (or, for boxes beyond 2^30 cells, per-(x, y)-column offsets with the
+ * points of a column sorted by z: FK_BUILD_COLUMNS), per-cloud layout, and
+ * optionally an attached, cell-sorted query list.
  * One handle must not be used from two streams at once.
  *
  * Reference interfaces replaced (reference = warriorsniu/FlashKNN_ExpRepo):
@@ -98,6 +100,13 @@ typedef enum fk_status {
 #define FK_BUILD_NO_SYNC 1 /* never read sizes back to the host: use the
                               reserved capacity; overflow is reported by
                               fk_grid_status() (graph-capture safe)         */
+#define FK_BUILD_COLUMNS 2 /* column layout even when the dense cell box
+                              fits (synchronising builds only).  A
+                              synchronising build whose dense boxes exceed
+                              2^30 cells takes this layout by itself: one
+                              table entry per (x, y) column, points sorted by
+                              z inside it; results are the same, lookups
+                              search the column                            */
 
 typedef struct fk_query_params {
   int32_t k;            /* neighbours per query, 1..FK_MAX_K               */
@@ -223,7 +232,9 @@ int fk_grid_status(fk_grid *grid, void *stream);
 int fk_grid_info(fk_grid *grid, int64_t *num_points, int32_t *num_clouds,
                  int64_t *num_cells, int64_t *num_queries);
 /* Device pointers of the build (for tests and the backward): sorted support
- * records float4 (x,y,z,id bits), cell start table int32[num_cells+1]. */
+ * records float4 (x,y,z,id bits), cell start table int32[num_cells+1] (one
+ * entry per cell; per (x, y) column in the column layout, FK_BUILD_COLUMNS,
+ * where fk_grid_info's num_cells counts columns). */
 int fk_grid_tables(fk_grid *grid, const void **sorted_records,
                    const int32_t **cell_start);
 

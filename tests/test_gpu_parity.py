@@ -102,6 +102,38 @@ def test_query_tiny_and_empty(oracle):
     assert idx0.shape == (0, 4)
 
 
+def test_query_large_sparse_box(oracle):
+    """Clusters spread over 60 m x 60 m x 10 m at 5 cm cells: a dense box of
+    ~3.3e8 cells (above 2^28, below the 2^30 cap) builds and matches; boxes
+    beyond the cap (one cloud, or the total over clouds) take the column
+    layout and match too."""
+    rng = np.random.default_rng(61)
+    centres = rng.random((6, 3)) * np.array([60.0, 60.0, 10.0])
+    xyz = np.concatenate([c + rng.normal(0, 0.15, (800, 3)) for c in centres] +
+                         [np.array([[0.01, 0.01, 0.01], [59.99, 59.99, 9.99]])])
+    xyz = xyz.astype(np.float32)
+    grid = np.floor(xyz / np.float32(0.05)).astype(np.int64)
+    span = grid.max(0) - grid.min(0) + 1
+    assert 1 << 28 < np.prod(span) < 1 << 30
+    for k in (1, 16):
+        ref_i, ref_d = oracle.flashknn_query(grid, None, xyz, k, 0)
+        knn = fk.FlashKNN(num_nbr=k, num_down=0)
+        idx = knn.query(_t(grid), None, _t(xyz))
+        _same(idx, knn.last_distances, ref_i, ref_d)
+    knn = fk.FlashKNN(num_nbr=8, num_down=0)
+    one = np.concatenate([grid, [[1100, 1100, 1100]]])
+    half = np.concatenate([grid, [[0, 0, 600]]])
+    two = np.concatenate([half, half + 3])
+    bt = np.repeat([0, 1], len(half))
+    box = lambda c: float(np.prod((c.max(0) - c.min(0) + 1).astype(np.float64)))
+    assert box(one) > 2.0 ** 30                                    # one cloud
+    assert box(half) < 2.0 ** 30 < 2 * box(half)                   # the total
+    for g, b in ((one, None), (two, bt)):
+        ref_i, ref_d = oracle.flashknn_query(g, b, None, 8, 0)
+        idx = knn.query(_t(g), None if b is None else _t(b), None)
+        _same(idx, knn.last_distances, ref_i, ref_d)
+
+
 def test_query_mode_validation():
     knn = fk.FlashKNN(num_nbr=4, num_down=0)
     g = _t(np.zeros((4, 3), np.int64))
@@ -205,6 +237,23 @@ def test_exact_other_queries_and_padding(oracle):
     idx, d = fk.knn(_t(s), _t(soff), 8, _t(q), _t(qoff))
     _same(idx, d, ref_i, ref_d)
     assert (idx[:300, 5:] == -1).all()
+
+
+def test_exact_engine_grown_cells_then_no_sync_attach(oracle):
+    """A synchronising build (explicit fine cell) grows the cell tables past
+    the engine's reserve; the next no-sync build + attach still fits."""
+    rng = np.random.default_rng(71)
+    s = rng.random((3000, 3), dtype=np.float32)
+    q = rng.random((700, 3), dtype=np.float32)
+    off, qoff = np.array([3000], np.int32), np.array([700], np.int32)
+    eng = fk.ExactKNN()
+    eng(_t(s), _t(off), 8)                                   # reserves, no-sync
+    idx, d = eng(_t(s), _t(off), 8, cell=torch.tensor([0.004]))   # 1.6e7 cells, sync
+    ref_i, ref_d = oracle.brute_knn(s, off, s, off, 8)
+    _same(idx, d, ref_i, ref_d)
+    idx, d = eng(_t(s), _t(off), 8, _t(q), _t(qoff))         # no-sync build + attach
+    ref_i, ref_d = oracle.brute_knn(s, off, q, qoff, 8)
+    _same(idx, d, ref_i, ref_d)
 
 
 def test_pointops_knn_query(oracle):
