@@ -154,7 +154,7 @@ int bind_device(fk_grid *g) {
 }
 
 int ensure_clouds(fk_grid *g, int64_t B) {
-  FK_CHECK(grow(g->bbox, g->cap_bbox, 6 * B, "bbox"));
+  FK_CHECK(grow(g->bbox, g->cap_bbox, box_words(B), "bbox"));
   FK_CHECK(grow(g->clouds, g->cap_clouds, B, "clouds"));
   FK_CHECK(grow(g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
   FK_CHECK(grow(g->cloud_acc, g->cap_acc, B, "per-cloud sums"));

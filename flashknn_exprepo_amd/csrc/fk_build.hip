@@ -194,11 +194,33 @@ __device__ __forceinline__ float float_of_ord(int i) {
   return __int_as_float(i >= 0 ? i : (i ^ 0x7fffffff));
 }
 
+// The copy of the per-cloud boxes a block accumulates into.
+__device__ __forceinline__ int32_t *box_copy(int32_t *bbox, int32_t B) {
+  return bbox + (int64_t)(blockIdx.x & (box_copies(B) - 1)) * box_stride(B);
+}
+
+// Union of the copies of cloud c's box -> b[6].
+__device__ __forceinline__ void box_read(const int32_t *bbox, int32_t B, int c,
+                                         int32_t *b) {
+  const int64_t stride = box_stride(B);
+  const int32_t *p = bbox + 6 * c;
+  b[0] = p[0]; b[1] = p[1]; b[2] = p[2]; b[3] = p[3]; b[4] = p[4]; b[5] = p[5];
+  for (int k = 1; k < box_copies(B); ++k) {
+    const int32_t *q = p + k * stride;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      b[a] = min(b[a], q[a]);
+      b[3 + a] = max(b[3 + a], q[3 + a]);
+    }
+  }
+}
+
 __global__ void k_init_bbox(int32_t *bbox, int32_t B) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= B) return;
-  bbox[6 * c + 0] = bbox[6 * c + 1] = bbox[6 * c + 2] = INT_MAX;
-  bbox[6 * c + 3] = bbox[6 * c + 4] = bbox[6 * c + 5] = INT_MIN;
+  const int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (e >= (int64_t)box_copies(B) * B) return;
+  int32_t *p = bbox + (e / B) * box_stride(B) + 6 * (e % B);
+  p[0] = p[1] = p[2] = INT_MAX;
+  p[3] = p[4] = p[5] = INT_MIN;
 }
 
 constexpr int64_t kCellLimit = 1ll << 30;  // |cell coordinate| bound
@@ -210,18 +232,31 @@ __device__ __forceinline__ bool cell_ok(int64_t v) {
 // cell = grid >> num_down (arithmetic shift = floor division by 2^num_down)
 __global__ void __launch_bounds__(256)
 k_cells_from_grid(const int64_t *grid, const int64_t *batch, int64_t n,
-                  int32_t num_down, int32_t B, int4 *pt_cell, int32_t *bbox,
+                  int32_t num_down, int32_t B, int4 *pt_cell, int32_t *bbox_all,
                   int32_t *scalars) {
+  int32_t *bbox = box_copy(bbox_all, B);
   Box box;
   box_init(box);
   const int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems;
+  // all loads first: the atomics below would otherwise keep the compiler
+  // from overlapping one item's loads with the previous item's work
+  int64_t c[kItems][3], bt[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
+    c[j][0] = c[j][1] = c[j][2] = bt[j] = 0;
+    if (i < n) {
+      c[j][0] = grid[3 * i]; c[j][1] = grid[3 * i + 1]; c[j][2] = grid[3 * i + 2];
+      bt[j] = batch ? batch[i] : 0;
+    }
+  }
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
     if (i < n) {
-      const int64_t gx = grid[3 * i] >> num_down, gy = grid[3 * i + 1] >> num_down,
-                    gz = grid[3 * i + 2] >> num_down;
-      const int64_t b = batch ? batch[i] : 0;
+      const int64_t gx = c[j][0] >> num_down, gy = c[j][1] >> num_down,
+                    gz = c[j][2] >> num_down;
+      const int64_t b = bt[j];
       if (b < 0 || b >= B || !cell_ok(gx) || !cell_ok(gy) || !cell_ok(gz)) {
         raise_status(scalars, FK_ERR_DOMAIN);
         pt_cell[i] = make_int4(0, 0, 0, -1);
@@ -253,23 +288,37 @@ __device__ __forceinline__ float scaled(float x, float inv) {
 
 __global__ void __launch_bounds__(256)
 k_cells_from_xyz(const float *xyz, const int32_t *offsets, int32_t B, int64_t n,
-                 const float *inv_cell, int4 *pt_cell, int32_t *bbox,
+                 const float *inv_cell, int4 *pt_cell, int32_t *bbox_all,
                  int32_t *scalars) {
+  int32_t *bbox = box_copy(bbox_all, B);
   Box box;
   box_init(box);
   const int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems;
+  // loads first (see k_cells_from_grid)
+  float p[kItems][3];
+  int cls[kItems];
+#pragma unroll
+  for (int j = 0; j < kItems; ++j) {
+    const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
+    p[j][0] = p[j][1] = p[j][2] = 0.f;
+    cls[j] = B;
+    if (i < n) {
+      p[j][0] = xyz[3 * i]; p[j][1] = xyz[3 * i + 1]; p[j][2] = xyz[3 * i + 2];
+      cls[j] = cloud_of(offsets, B, i);
+    }
+  }
 #pragma unroll
   for (int j = 0; j < kItems; ++j) {
     const int64_t i = tile + threadIdx.x + (int64_t)j * blockDim.x;
     if (i < n) {
-      const int cl = cloud_of(offsets, B, i);
+      const int cl = cls[j];
       bool ok = cl < B;
       float fx = 0.f, fy = 0.f, fz = 0.f;
       if (ok) {
         const float inv = inv_cell[cl];
-        fx = floorf(scaled(xyz[3 * i], inv));
-        fy = floorf(scaled(xyz[3 * i + 1], inv));
-        fz = floorf(scaled(xyz[3 * i + 2], inv));
+        fx = floorf(scaled(p[j][0], inv));
+        fy = floorf(scaled(p[j][1], inv));
+        fz = floorf(scaled(p[j][2], inv));
         // rejects NaN as well
         ok = fabsf(fx) < (float)kCellLimit && fabsf(fy) < (float)kCellLimit &&
              fabsf(fz) < (float)kCellLimit;
@@ -289,7 +338,8 @@ k_cells_from_xyz(const float *xyz, const int32_t *offsets, int32_t B, int64_t n,
 // Float bounding box of every cloud (ord_of-mapped ints in bbox[6c..6c+5]).
 __global__ void __launch_bounds__(256)
 k_cloud_bounds(const float *xyz, const int32_t *offsets, int32_t B, int64_t n,
-               int32_t *bbox) {
+               int32_t *bbox_all) {
+  int32_t *bbox = box_copy(bbox_all, B);
   Box box;
   box_init(box);
   const int64_t tile = (int64_t)blockIdx.x * blockDim.x * kItems;
@@ -315,7 +365,8 @@ __global__ void k_auto_cell(const int32_t *bbox, const int32_t *offsets,
                             float scale, float *side_out, float *inv_out) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= B) return;
-  const int32_t *b = bbox + 6 * c;
+  int32_t b[6];
+  box_read(bbox, B, c, b);
   const int64_t n = offsets[c] - (c ? offsets[c - 1] : 0);
   double side = 1.0;
   if (n > 0 && b[0] <= b[3]) {
@@ -359,7 +410,8 @@ k_layout(const int32_t *bbox, int32_t B, const int32_t *offsets,
     int64_t cells = 0;
     int nx = 0, ny = 0, nz = 0, mnx = 0, mny = 0, mnz = 0;
     if (c < B) {
-      const int32_t *b = bbox + 6 * c;
+      int32_t b[6];
+      box_read(bbox, B, c, b);
       if (b[0] <= b[3]) {
         mnx = b[0]; mny = b[1]; mnz = b[2];
         const int64_t ex = (int64_t)b[3] - b[0] + 1, ey = (int64_t)b[4] - b[1] + 1,
@@ -442,8 +494,44 @@ __device__ __forceinline__ int32_t wave_rank(int32_t *counts, int32_t lin,
   return rank;
 }
 
-// Linear cell id and in-cell rank of every point.
-__global__ void __launch_bounds__(kBlock)
+// In-entry rank aggregated over a whole block: an LDS hash table groups the
+// block's lanes by table entry, each distinct entry takes one global atomic
+// for its lanes, and a lane's rank is that base plus its order in the LDS
+// count.  Coarse grids put thousands of points into one cell, and atomics on
+// one address from different CUs serialise; one per block and cell instead
+// of one per wave took the adaptive path's 829k-point coarse-level build
+// from 73 to 20 us.  Every
+// thread of the block calls it; lanes with act = false get 0.
+constexpr int kRankBlock = 1024;
+constexpr int kRankSlots = 2 * kRankBlock;  // > distinct keys: probing ends
+
+__device__ __forceinline__ int32_t block_rank(int32_t *counts, int32_t lin,
+                                              bool act) {
+  __shared__ int32_t key[kRankSlots], cnt[kRankSlots], base[kRankSlots];
+  for (int j = threadIdx.x; j < kRankSlots; j += blockDim.x) {
+    key[j] = -1;  // table entries are >= 0
+    cnt[j] = 0;
+  }
+  __syncthreads();
+  int slot = 0, local = 0;
+  if (act) {
+    slot = (int)(((uint32_t)lin * 2654435761u) >> 21);  // 11 bits
+    for (;;) {
+      const int old = atomicCAS(&key[slot], -1, lin);
+      if (old == -1 || old == lin) break;
+      slot = (slot + 1) & (kRankSlots - 1);
+    }
+    local = atomicAdd(&cnt[slot], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kRankSlots; j += blockDim.x)
+    if (key[j] >= 0) base[j] = atomicAdd(counts + key[j], cnt[j]);
+  __syncthreads();
+  return act ? base[slot] + local : 0;
+}
+
+// Table entry and in-entry rank of every point.
+__global__ void __launch_bounds__(kRankBlock)
 k_count(const int4 *pt_cell, int64_t n, const CloudInfo *clouds,
         int32_t *counts, int32_t *pt_lin, int32_t *pt_rank,
         const int32_t *scalars) {
@@ -456,7 +544,7 @@ k_count(const int4 *pt_cell, int64_t n, const CloudInfo *clouds,
     lin = lin_of(ci, c.x - ci.minx, c.y - ci.miny, c.z - ci.minz);
     pt_lin[i] = lin;
   }
-  const int32_t rank = wave_rank(counts, lin, act);
+  const int32_t rank = block_rank(counts, lin, act);
   if (act) pt_rank[i] = rank;
 }
 
@@ -518,21 +606,39 @@ k_scan_partials(int32_t *part, int32_t nparts) {
   }
 }
 
+// LDS index of tile element i, padded by one word per 16 so that both the
+// striped (i = j * kBlock + t) and the blocked (i = t * kScanItems + j)
+// accesses of a wave are free of bank conflicts.
+constexpr __host__ __device__ int scan_pad(int i) { return i + (i >> 4); }
+
+// The tile goes through LDS: loaded and stored striped (consecutive lanes,
+// consecutive words: coalesced), scanned blocked (kScanItems consecutive
+// elements per thread).  Reading and writing each thread's own 16
+// consecutive words straight from memory made every wave access 64
+// separate 64-byte strides (~1.8 TB/s on a 38M-entry table).
 __global__ void __launch_bounds__(kBlock)
 k_scan_apply(const int32_t *in, const int32_t *n_dev, int64_t n_fixed,
              int32_t extra, const int32_t *part, int32_t *out) {
+  __shared__ int tile[scan_pad(kScanTile)];
   __shared__ int wsum[kBlock / 64];
   const int64_t N = (n_dev ? (int64_t)(*n_dev) : n_fixed) + extra;
-  const int64_t base = (int64_t)blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int i = j * kBlock + t;
+    const int64_t idx = base + i;
+    tile[scan_pad(i)] = idx < N ? in[idx] : 0;
+  }
+  __syncthreads();
   int v[kScanItems];
   int s = 0;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    const int64_t idx = base + j;
-    v[j] = idx < N ? in[idx] : 0;
+    v[j] = tile[scan_pad(t * kScanItems + j)];
     s += v[j];
   }
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lane = t & 63, w = t >> 6;
   int incl = s;
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -546,9 +652,15 @@ k_scan_apply(const int32_t *in, const int32_t *n_dev, int64_t n_fixed,
   int run = before + incl - s;
 #pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    const int64_t idx = base + j;
-    if (idx <= N) out[idx] = run;
+    tile[scan_pad(t * kScanItems + j)] = run;
     run += v[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) {
+    const int i = j * kBlock + t;
+    const int64_t idx = base + i;
+    if (idx <= N) out[idx] = tile[scan_pad(i)];
   }
 }
 
@@ -713,7 +825,7 @@ hipError_t launch_build_cells(const BuildBuffers &b, const float *xyz,
                               int64_t n, int32_t B, int32_t num_down,
                               hipStream_t s) {
   (void)xyz;
-  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for((int64_t)box_copies(B) * B)), dim3(kBlock), 0, s,
                      b.bbox, B);
   if (n > 0)
     hipLaunchKernelGGL(k_cells_from_grid, dim3(blocks_for(n, kBlock * kItems)),
@@ -725,7 +837,7 @@ hipError_t launch_build_cells(const BuildBuffers &b, const float *xyz,
 hipError_t launch_build_xyz_cells(const BuildBuffers &b, const float *xyz,
                                   const int32_t *offsets, int32_t B, int64_t n,
                                   const float *inv_cell, hipStream_t s) {
-  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s,
+  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for((int64_t)box_copies(B) * B)), dim3(kBlock), 0, s,
                      b.bbox, B);
   if (n > 0)
     hipLaunchKernelGGL(k_cells_from_xyz, dim3(blocks_for(n, kBlock * kItems)),
@@ -738,7 +850,7 @@ hipError_t launch_auto_cell(int32_t *bbox, const float *xyz,
                             const int32_t *offsets, int32_t B, int64_t n,
                             const int32_t *k_per_cloud, int32_t k, float scale,
                             float *side_out, float *inv_out, hipStream_t s) {
-  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for(B)), dim3(kBlock), 0, s, bbox, B);
+  hipLaunchKernelGGL(k_init_bbox, dim3(blocks_for((int64_t)box_copies(B) * B)), dim3(kBlock), 0, s, bbox, B);
   if (n > 0)
     hipLaunchKernelGGL(k_cloud_bounds, dim3(blocks_for(n, kBlock * kItems)),
                        dim3(kBlock), 0, s, xyz, offsets, B, n, bbox);
@@ -762,7 +874,7 @@ hipError_t launch_count_scan(const BuildBuffers &b, int64_t n,
   hipLaunchKernelGGL(k_fill_dev, dim3(nfill), dim3(kBlock), 0, s, b.counts, 0,
                      b.scalars + kTotalCells, 0);
   if (n > 0)
-    hipLaunchKernelGGL(k_count, dim3(blocks_for(n)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_count, dim3(blocks_for(n, kRankBlock)), dim3(kRankBlock), 0, s,
                        b.pt_cell, n, b.clouds, b.counts, b.pt_lin, b.pt_rank,
                        b.scalars);
   return launch_scan(b.counts, b.cell_start, b.scalars + kTotalCells, 0,
