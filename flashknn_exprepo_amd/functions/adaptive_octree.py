@@ -41,6 +41,32 @@ _CHUNK = 1024   # points per chunk of the per-batch minimum
 _SPARE = 4096   # spare rows that absorb the atomics of unused lanes
 
 
+def walk_levels(c: torch.Tensor, shifts: torch.Tensor, lo: int, hi: int):
+    """The coarse-to-fine level walk over consecutive levels at once.
+
+    ``c`` [L, n]: candidate counts at levels shifts[0] > shifts[1] > ...
+    (coarse first).  Returns (sel int16 [n], active bool [n]) equal to the
+    walk ``active = c[0] > hi``, then per level i >= 1
+    ``use = active & (c[i] >= lo); sel[use] = shifts[i];
+    active = use & (c[i] > hi)``, starting from sel = shifts[0]: a point is
+    active after level i iff c[0] > hi and c[j] > hi, c[j] >= lo for
+    1 <= j <= i (a prefix AND, i.e. a running minimum), and it takes the last
+    level i >= 1 where it was active before and reached lo."""
+    ge = c >= lo
+    keep = c > hi
+    keep[1:] &= ge[1:]
+    run = keep.to(torch.int32).cummin(0).values
+    sh = shifts.to(torch.int16)
+    if c.shape[0] > 1:
+        use = run[:-1].bool() & ge[1:]
+        rank = torch.arange(1, c.shape[0], device=c.device, dtype=torch.int32)[:, None]
+        last = (use.to(torch.int32) * rank).amax(0)
+        sel = sh[last.long()]
+    else:
+        sel = sh[0].expand(c.shape[1]).clone()
+    return sel, run[-1].bool()
+
+
 def batch_min(v: torch.Tensor, batch: torch.Tensor, B: int) -> torch.Tensor:
     """Per-batch minimum of v (int64 [N, 3]) -> int64 [B, 3], no host sync.
 
@@ -259,18 +285,24 @@ class AdaptiveNeighborhoodFlashKNN:
                 counts[s] = c_rep[i]
                 built[s] = comb
         reserved = {}  # level -> its grid was reserved (no-sync build and attach)
+        first_sep = max_shift  # first level built on its own
         if Lc:
             reserved[-1] = True
-        for s in range(max_shift, -1, -1):
-            if s not in counts:
-                gr = self._level_grid(s)
-                cap = box_cells(s) + 64
-                no_sync = cap <= _RESERVE_CAP
-                if no_sync:  # sized from the host-side spans: no read-back
-                    gr.reserve(n, cap, B, n)
-                gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
-                counts[s], built[s] = gr.point_candidates(1), gr
-                reserved[s] = no_sync
+            sel, active = walk_levels(c_rep, shifts, lo_f, hi_f)
+            first_sep = small[-1] - 1
+            # a finer level no point can select changes nothing; read the mask
+            # back only before a separate (large) box
+            if first_sep >= 0 and not bool(active.any()):
+                first_sep = -1
+        for s in range(first_sep, -1, -1):
+            gr = self._level_grid(s)
+            cap = box_cells(s) + 64
+            no_sync = cap <= _RESERVE_CAP
+            if no_sync:  # sized from the host-side spans: no read-back
+                gr.reserve(n, cap, B, n)
+            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
+            counts[s], built[s] = gr.point_candidates(1), gr
+            reserved[s] = no_sync
             c = counts[s]
             if active is None:
                 active = c > hi_f
@@ -278,9 +310,7 @@ class AdaptiveNeighborhoodFlashKNN:
                 use = active & (c >= lo_f)
                 sel = torch.where(use, torch.full_like(sel, s), sel)
                 active = use & (c > hi_f)
-            # a finer level no point can select changes nothing; read the mask
-            # back only before a separate (large) box
-            if s > 0 and (s - 1) not in counts and not bool(active.any()):
+            if s > 0 and not bool(active.any()):
                 break
         torch.cuda.synchronize(dev)
         t_build = (time.perf_counter() - t0) * 1e3

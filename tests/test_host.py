@@ -116,3 +116,26 @@ def test_quotient32_correctly_rounded():
     for size in (0.06, 0.04, 0.24, 1e-3, 7.0):
         got = quotient32(torch.from_numpy(x), size).numpy()
         assert np.array_equal(got.view(np.uint32), (x / np.float32(size)).view(np.uint32)), size
+
+
+@pytest.mark.parametrize("lo,hi", [(32, 128), (16, 16), (100, 40)])
+def test_walk_levels_equals_the_level_loop(lo, hi):
+    """The vectorised coarse-to-fine walk of the adaptive neighbourhood equals
+    the level-by-level loop it replaces (including lo > hi)."""
+    from flashknn_exprepo_amd.functions.adaptive_octree import walk_levels
+    rng = np.random.default_rng(lo * 7 + hi)
+    for L in (1, 2, 5, 9):
+        shifts = torch.arange(11, 11 - L, -1)
+        # counts shrink towards finer levels, with noise, hitting both bounds
+        base = rng.integers(0, 400, 5000)
+        c = np.stack([base >> (11 - int(s)) // 2 for s in shifts]) + rng.integers(0, 60, (L, 5000))
+        c = torch.from_numpy(np.sort(c, axis=0)[::-1].copy()).to(torch.int32)
+        c[:, :50] = torch.from_numpy(rng.integers(0, 300, (L, 50))).to(torch.int32)
+        sel = torch.full((5000,), int(shifts[0]), dtype=torch.int16)
+        active = c[0] > hi
+        for i in range(1, L):
+            use = active & (c[i] >= lo)
+            sel = torch.where(use, torch.full_like(sel, int(shifts[i])), sel)
+            active = use & (c[i] > hi)
+        s2, a2 = walk_levels(c, shifts, lo, hi)
+        assert torch.equal(s2, sel) and torch.equal(a2, active), L
