@@ -139,3 +139,23 @@ def test_walk_levels_equals_the_level_loop(lo, hi):
             active = use & (c[i] > hi)
         s2, a2 = walk_levels(c, shifts, lo, hi)
         assert torch.equal(s2, sel) and torch.equal(a2, active), L
+
+
+@pytest.mark.parametrize("n,B,contiguous", [(1, 1, True), (3000, 1, True), (5000, 4, True),
+                                            (5000, 4, False), (70000, 9, True), (2500, 6, True)])
+def test_batch_min_max_cpu(n, B, contiguous):
+    """Per-batch minimum / maximum (torch ops, any device) against numpy,
+    including chunk padding, interleaved batches and batches without points."""
+    from flashknn_exprepo_amd.functions.adaptive_octree import batch_max, batch_min
+    rng = np.random.default_rng(n * 3 + B)
+    b = rng.integers(0, B - 1 if B == 6 else B, n)          # B == 6: last batch empty
+    if contiguous:
+        b.sort()
+    v = rng.integers(-10**12, 10**12, (n, 3))
+    lo = np.full((B, 3), np.iinfo(np.int64).max, np.int64)
+    hi = np.full((B, 3), np.iinfo(np.int64).min, np.int64)
+    np.minimum.at(lo, b, v)
+    np.maximum.at(hi, b, v)
+    tv, tb = torch.from_numpy(v.astype(np.int64)), torch.from_numpy(b.astype(np.int64))
+    assert np.array_equal(batch_min(tv, tb, B).numpy(), lo)
+    assert np.array_equal(batch_max(tv, tb, B).numpy(), hi)
