@@ -21,6 +21,13 @@ namespace {
 
 constexpr int kBlock = 256;
 
+__device__ __forceinline__ int lane_value(int v, int j) {
+  return __builtin_amdgcn_readlane(v, j);
+}
+__device__ __forceinline__ float lane_value(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
 inline unsigned nblocks(int64_t n) {
   int64_t b = (n + kBlock - 1) / kBlock;
   return (unsigned)(b < 1 ? 1 : b);
@@ -129,47 +136,43 @@ __global__ void k_dist_bw_q(const float *q, int64_t nq, const float *p,
   gq[3 * i + 2] = az;
 }
 
-__global__ void k_dist_bw_p(const float *q, const float *p, int64_t ns,
-                            int32_t k, const float *g, const int32_t *start,
-                            const int32_t *slots, float *gp) {
-  const int64_t m = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (m >= ns) return;
+// One wave per support m: each 64-entry chunk of its (ascending) segment is
+// loaded one entry per lane, every lane forms its entry's term, and the terms
+// are added in entry order by reading them back lane by lane, so the sum is
+// the same as a sequential loop's (a thread per support walked ~50-70
+// dependent entry loads: ~45 us whatever the size).
+__global__ void __launch_bounds__(256)
+k_dist_bw_p(const float *q, const float *p, int64_t ns, int32_t k,
+            const float *g, const int32_t *start, const int32_t *slots,
+            float *gp) {
+  const int64_t m = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (m >= ns) return;  // wave-uniform
   const float px = p[3 * m], py = p[3 * m + 1], pz = p[3 * m + 2];
   float ax = 0.f, ay = 0.f, az = 0.f;
   const int32_t b = start[m], end = start[m + 1];
-  int32_t t = b;
-  // loads of eight entries in flight; the sum still runs in entry order
-  for (; t + 8 <= end; t += 8) {
-    int32_t e[8];
-    float c[8], dx[8], dy[8], dz[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) e[u] = slots[t + u];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int64_t i = e[u] / k;
-      c[u] = 2.f * g[e[u]];
-      dx[u] = q[3 * i] - px;
-      dy[u] = q[3 * i + 1] - py;
-      dz[u] = q[3 * i + 2] - pz;
+  for (int32_t c0 = b; c0 < end; c0 += 64) {
+    const int n = end - c0 < 64 ? end - c0 : 64;
+    float tx = 0.f, ty = 0.f, tz = 0.f;
+    if (lane < n) {
+      const int32_t e = slots[c0 + lane];
+      const int64_t i = e / k;
+      const float c = 2.f * g[e];
+      tx = c * (q[3 * i] - px);
+      ty = c * (q[3 * i + 1] - py);
+      tz = c * (q[3 * i + 2] - pz);
     }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      ax -= c[u] * dx[u];
-      ay -= c[u] * dy[u];
-      az -= c[u] * dz[u];
+    for (int u = 0; u < n; ++u) {
+      ax -= lane_value(tx, u);
+      ay -= lane_value(ty, u);
+      az -= lane_value(tz, u);
     }
   }
-  for (; t < end; ++t) {
-    const int32_t e = slots[t];
-    const int64_t i = e / k;
-    const float c = 2.f * g[e];
-    ax -= c * (q[3 * i] - px);
-    ay -= c * (q[3 * i + 1] - py);
-    az -= c * (q[3 * i + 2] - pz);
+  if (lane == 0) {
+    gp[3 * m] = ax;
+    gp[3 * m + 1] = ay;
+    gp[3 * m + 2] = az;
   }
-  gp[3 * m] = ax;
-  gp[3 * m + 1] = ay;
-  gp[3 * m + 2] = az;
 }
 
 // out[i, c] = sum_j w[i,j] feat[idx[i,j], c]  (thread per (i, c)).
@@ -190,32 +193,52 @@ __global__ void k_idw_fwd(const float *feat, int64_t ns, int32_t C,
 }
 
 // grad_feat[m, c] = sum over segment m (entry order) of w[e] grad_out[e/k, c].
-__global__ void k_idw_bw_feat(int64_t ns, int32_t C, const float *w, int32_t k,
-                              const float *gout, const int32_t *start,
-                              const int32_t *slots, float *gfeat) {
-  const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  if (t >= ns * C) return;
-  const int64_t m = t / C;
-  const int c = (int)(t - m * C);
+// grad_feat[m, c] = sum over the entries e of support m (ascending) of
+// w[e] * grad_out[e / k, c].  One wave per (support, 64-channel chunk), lane
+// = channel: the segment's entry ids and weights are loaded 64 at a time
+// (one per lane) and broadcast, so the loop only gathers grad_out rows
+// (coalesced over the channels), eight in flight; the sum runs in entry
+// order.
+__global__ void __launch_bounds__(256)
+k_idw_bw_feat(int64_t ns, int32_t C, const float *w, int32_t k,
+              const float *gout, const int32_t *start, const int32_t *slots,
+              float *gfeat) {
+  const int64_t wid = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  const int chunks = (C + 63) >> 6;
+  const int64_t m = wid / chunks;
+  if (m >= ns) return;  // wave-uniform
+  const int c = (int)(wid - m * chunks) * 64 + lane;
+  const bool cok = c < C;
   float acc = 0.f;
   const int32_t b = start[m], end = start[m + 1];
-  int32_t s = b;
-  for (; s + 8 <= end; s += 8) {  // eight gathers in flight, summed in order
-    float wv[8], gv[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int32_t e = slots[s + u];
-      wv[u] = w[e];
-      gv[u] = gout[(int64_t)(e / k) * C + c];
+  for (int32_t c0 = b; c0 < end; c0 += 64) {
+    const int n = end - c0 < 64 ? end - c0 : 64;
+    int32_t row = 0;  // grad_out row of the lane's entry
+    float we = 0.f;
+    if (lane < n) {
+      const int32_t e = slots[c0 + lane];
+      row = e / k;
+      we = w[e];
     }
+    int u = 0;
+    for (; u + 8 <= n; u += 8) {
+      float gv[8], wv[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) acc += wv[u] * gv[u];
+      for (int v = 0; v < 8; ++v) {
+        const int64_t r = lane_value(row, u + v);
+        wv[v] = lane_value(we, u + v);
+        gv[v] = cok ? gout[r * C + c] : 0.f;
+      }
+#pragma unroll
+      for (int v = 0; v < 8; ++v) acc += wv[v] * gv[v];
+    }
+    for (; u < n; ++u) {
+      const int64_t r = lane_value(row, u);
+      acc += lane_value(we, u) * (cok ? gout[r * C + c] : 0.f);
+    }
   }
-  for (; s < end; ++s) {
-    const int32_t e = slots[s];
-    acc += w[e] * gout[(int64_t)(e / k) * C + c];
-  }
-  gfeat[t] = acc;
+  if (cok) gfeat[m * C + c] = acc;
 }
 
 // grad_w[i, j] = <grad_out[i, :], feat[idx[i,j], :]>: 16 lanes per entry
@@ -262,10 +285,6 @@ __device__ __forceinline__ float wave_sum(float v) {
 constexpr int kGather = 8;  // neighbour rows in flight per lane (forward)
 
 // v of lane j, j wave-uniform (a scalar read, no LDS round trip)
-__device__ __forceinline__ int lane_value(int v, int j) {
-  return __builtin_amdgcn_readlane(v, j);
-}
-
 __device__ __forceinline__ int bitrev4(int v) {
   return ((v & 1) << 3) | ((v & 2) << 1) | ((v & 4) >> 1) | ((v & 8) >> 3);
 }
@@ -467,7 +486,7 @@ hipError_t launch_dist_backward(const float *q, int64_t nq, const float *p,
     const int32_t *sorted = nullptr;
     hipError_t err = build_reverse_csr(idx, E, ns, w, &sorted, s);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_dist_bw_p, dim3(nblocks(ns)), dim3(kBlock), 0, s, q, p,
+    hipLaunchKernelGGL(k_dist_bw_p, dim3(nblocks(ns * 64)), dim3(kBlock), 0, s, q, p,
                        ns, k, grad_d, w.start, sorted, grad_p);
   }
   return hipGetLastError();
@@ -497,7 +516,7 @@ hipError_t launch_idw_backward(const float *feat, int64_t ns, int32_t C,
     const int32_t *sorted = nullptr;
     hipError_t err = build_reverse_csr(idx, E, ns, wsp, &sorted, s);
     if (err != hipSuccess) return err;
-    hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * C)), dim3(kBlock), 0, s,
+    hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * ((C + 63) / 64) * 64)), dim3(kBlock), 0, s,
                        ns, C, w, k, grad_out, wsp.start, sorted, grad_feat);
   }
   return hipGetLastError();
@@ -541,10 +560,10 @@ hipError_t launch_interp_backward(const float *q, int64_t nq, const float *p,
     hipError_t err = build_reverse_csr(idx, E, ns, wsp, &sorted, s);
     if (err != hipSuccess) return err;
     if (grad_p)
-      hipLaunchKernelGGL(k_dist_bw_p, dim3(nblocks(ns)), dim3(kBlock), 0, s, q, p, ns,
+      hipLaunchKernelGGL(k_dist_bw_p, dim3(nblocks(ns * 64)), dim3(kBlock), 0, s, q, p, ns,
                          k, gd, wsp.start, sorted, grad_p);
     if (grad_feat && ns * C > 0)
-      hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * C)), dim3(kBlock), 0, s,
+      hipLaunchKernelGGL(k_idw_bw_feat, dim3(nblocks(ns * ((C + 63) / 64) * 64)), dim3(kBlock), 0, s,
                          ns, C, w, k, grad_out, wsp.start, sorted, grad_feat);
   }
   return hipGetLastError();

@@ -494,42 +494,6 @@ __device__ __forceinline__ int32_t wave_rank(int32_t *counts, int32_t lin,
   return rank;
 }
 
-// In-entry rank aggregated over a whole block: an LDS hash table groups the
-// block's lanes by table entry, each distinct entry takes one global atomic
-// for its lanes, and a lane's rank is that base plus its order in the LDS
-// count.  Coarse grids put thousands of points into one cell, and atomics on
-// one address from different CUs serialise; one per block and cell instead
-// of one per wave took the adaptive path's 829k-point coarse-level build
-// from 73 to 20 us.  Every
-// thread of the block calls it; lanes with act = false get 0.
-constexpr int kRankBlock = 1024;
-constexpr int kRankSlots = 2 * kRankBlock;  // > distinct keys: probing ends
-
-__device__ __forceinline__ int32_t block_rank(int32_t *counts, int32_t lin,
-                                              bool act) {
-  __shared__ int32_t key[kRankSlots], cnt[kRankSlots], base[kRankSlots];
-  for (int j = threadIdx.x; j < kRankSlots; j += blockDim.x) {
-    key[j] = -1;  // table entries are >= 0
-    cnt[j] = 0;
-  }
-  __syncthreads();
-  int slot = 0, local = 0;
-  if (act) {
-    slot = (int)(((uint32_t)lin * 2654435761u) >> 21);  // 11 bits
-    for (;;) {
-      const int old = atomicCAS(&key[slot], -1, lin);
-      if (old == -1 || old == lin) break;
-      slot = (slot + 1) & (kRankSlots - 1);
-    }
-    local = atomicAdd(&cnt[slot], 1);
-  }
-  __syncthreads();
-  for (int j = threadIdx.x; j < kRankSlots; j += blockDim.x)
-    if (key[j] >= 0) base[j] = atomicAdd(counts + key[j], cnt[j]);
-  __syncthreads();
-  return act ? base[slot] + local : 0;
-}
-
 // Table entry and in-entry rank of every point.
 __global__ void __launch_bounds__(kRankBlock)
 k_count(const int4 *pt_cell, int64_t n, const CloudInfo *clouds,

@@ -84,6 +84,44 @@ __device__ __forceinline__ void raise_status(int32_t *scalars, int code) {
   atomicCAS(scalars + kStatus, 0, code);
 }
 
+// Rank within a counter (counts[v]++ for every active lane, returning the
+// old value) aggregated over a whole block of kRankBlock threads: an LDS
+// hash table groups the block's lanes by value, each distinct value takes
+// one global atomic for its lanes, and a lane's rank is that base plus its
+// order in the LDS count (ranks within a value are unique, in no fixed
+// order).  Atomics on one address from different CUs serialise; coarse grids
+// put thousands of points into one cell (one atomic per block and cell
+// instead of one per wave took the adaptive path's 829k-point coarse-level
+// count from 73 to 20 us).  Every thread of the block calls it; values are
+// >= 0; lanes with act = false get 0.
+constexpr int kRankBlock = 1024;
+constexpr int kRankSlots = 2 * kRankBlock;  // > distinct values: probing ends
+
+__device__ __forceinline__ int32_t block_rank(int32_t *counts, int32_t v,
+                                              bool act) {
+  __shared__ int32_t key[kRankSlots], cnt[kRankSlots], base[kRankSlots];
+  for (int j = threadIdx.x; j < kRankSlots; j += blockDim.x) {
+    key[j] = -1;
+    cnt[j] = 0;
+  }
+  __syncthreads();
+  int slot = 0, local = 0;
+  if (act) {
+    slot = (int)(((uint32_t)v * 2654435761u) >> 21);  // 11 bits
+    for (;;) {
+      const int old = atomicCAS(&key[slot], -1, v);
+      if (old == -1 || old == v) break;
+      slot = (slot + 1) & (kRankSlots - 1);
+    }
+    local = atomicAdd(&cnt[slot], 1);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < kRankSlots; j += blockDim.x)
+    if (key[j] >= 0) base[j] = atomicAdd(counts + key[j], cnt[j]);
+  __syncthreads();
+  return act ? base[slot] + local : 0;
+}
+
 __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   // Workgroups are dispatched round-robin over the 8 XCDs (bid % 8).  Give
   // each XCD one contiguous range of logical tiles so that neighbouring
