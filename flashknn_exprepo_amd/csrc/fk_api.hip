@@ -80,6 +80,9 @@ struct fk_grid {
   int32_t *h_pinned = nullptr;
   // per cloud
   int64_t cap_clouds = 0;
+  // clouds every per-cloud buffer holds (their capacities differ: the boxes
+  // take box_words(B) words, the others one entry per cloud)
+  int64_t max_clouds = 0;
   int32_t *bbox = nullptr;
   int64_t cap_bbox = 0;
   CloudInfo *clouds = nullptr;
@@ -158,6 +161,7 @@ int ensure_clouds(fk_grid *g, int64_t B) {
   FK_CHECK(grow(g->clouds, g->cap_clouds, B, "clouds"));
   FK_CHECK(grow(g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
   FK_CHECK(grow(g->cloud_acc, g->cap_acc, B, "per-cloud sums"));
+  if (B > g->max_clouds) g->max_clouds = B;
   return FK_OK;
 }
 
@@ -430,7 +434,7 @@ int fk_grid_build_cells(fk_grid *g, const float *xyz, const int64_t *grid_coord,
   if (nosync && (flags & FK_BUILD_COLUMNS))
     return fail(FK_ERR_INVALID_ARG, "FK_BUILD_COLUMNS needs a synchronising build");
   if (nosync) {
-    if (n > g->cap_points || num_clouds > g->cap_clouds || !g->counts)
+    if (n > g->cap_points || num_clouds > g->max_clouds || !g->counts)
       return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs fk_grid_reserve "
                                    "of at least n points and num_clouds clouds");
   } else {
@@ -462,7 +466,7 @@ int fk_grid_build_xyz(fk_grid *g, const float *xyz, const int32_t *offsets,
   if (nosync && (flags & FK_BUILD_COLUMNS))
     return fail(FK_ERR_INVALID_ARG, "FK_BUILD_COLUMNS needs a synchronising build");
   if (nosync) {
-    if (n > g->cap_points || num_clouds > g->cap_clouds || !g->counts)
+    if (n > g->cap_points || num_clouds > g->max_clouds || !g->counts)
       return fail(FK_ERR_CAPACITY, "FK_BUILD_NO_SYNC needs fk_grid_reserve "
                                    "of at least n points and num_clouds clouds");
   } else {
@@ -716,7 +720,7 @@ int fk_grid_auto_cell(fk_grid *g, const float *xyz, const int32_t *offsets,
   if (n < 0 || num_clouds < 1 || !offsets || !inv_out || (n > 0 && !xyz) ||
       (!k_per_cloud && (k < 1 || k > FK_MAX_K)) || !(scale > 0.f))
     return fail(FK_ERR_INVALID_ARG, "fk_grid_auto_cell: bad arguments");
-  if (num_clouds > g->cap_clouds) {
+  if (num_clouds > g->max_clouds) {
     if (g->reserved)
       return fail(FK_ERR_CAPACITY, "fk_grid_auto_cell: more clouds than reserved");
     FK_CHECK(ensure_clouds(g, num_clouds));

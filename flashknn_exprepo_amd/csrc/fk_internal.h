@@ -64,8 +64,12 @@ __host__ __device__ __forceinline__ int box_copies(int64_t B) { return B <= 1024
 __host__ __device__ __forceinline__ int64_t box_stride(int64_t B) {
   return (6 * B + 31) / 32 * 32;
 }
+// Words the boxes of B clouds need, non-decreasing in B: a buffer sized for
+// B clouds (a reserve) also holds the 16 copies of any smaller count.
 __host__ __device__ __forceinline__ int64_t box_words(int64_t B) {
-  return box_copies(B) * box_stride(B);
+  const int64_t small = 16 * box_stride(1024);
+  if (B <= 1024) return box_copies(B) * box_stride(B);
+  return box_stride(B) > small ? box_stride(B) : small;
 }
 
 // Table entries of a cloud's box.

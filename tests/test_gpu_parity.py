@@ -256,6 +256,36 @@ def test_exact_engine_grown_cells_then_no_sync_attach(oracle):
     _same(idx, d, ref_i, ref_d)
 
 
+def test_reserved_cloud_count_is_exact(oracle):
+    """A no-sync build may use up to the reserved number of clouds (their box
+    buffers included) and is refused beyond it; a handle reserved for many
+    clouds builds fewer (16 box copies each) correctly."""
+    from flashknn_exprepo_amd.grid import Grid
+    rng = np.random.default_rng(72)
+    n = 6000
+    grid = rng.integers(0, 4, (n, 3))
+    g = Grid()
+    g.reserve(n, 1 << 16, 10, n)
+    for B in (1, 10):
+        batch = rng.integers(0, B, n)
+        g.build_cells(_t(grid), _t(batch), None, num_clouds=B, no_sync=True)
+        g.status()
+        idx, d = g.query(8)
+        ref_i, ref_d = oracle.flashknn_query(grid, batch, None, 8, 0)
+        _same(idx, d, ref_i, ref_d)
+    with pytest.raises(fk.FlashKNNError):
+        g.build_cells(_t(grid), _t(rng.integers(0, 50, n)), None, num_clouds=50,
+                      no_sync=True)
+    g2 = Grid()
+    g2.reserve(n, 1 << 18, 2000, n)
+    batch = rng.integers(0, 1000, n)
+    g2.build_cells(_t(grid), _t(batch), None, num_clouds=1000, no_sync=True)
+    g2.status()
+    idx, d = g2.query(8)
+    ref_i, ref_d = oracle.flashknn_query(grid, batch, None, 8, 0)
+    _same(idx, d, ref_i, ref_d)
+
+
 def test_pointops_knn_query(oracle):
     rng = np.random.default_rng(11)
     s = rng.random((4000, 3), dtype=np.float32)
