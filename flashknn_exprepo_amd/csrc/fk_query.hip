@@ -726,6 +726,16 @@ __device__ __forceinline__ bool certified(double gap, float eff, float inv,
 // sqrt(k / found), between 1.25 and the doubling 2 + 1/Ro (found = 0:
 // doubling): scanned-out sparse surroundings (LiDAR's far range) would
 // otherwise double into shells of thousands of empty columns.
+// Shell growth while fewer than k candidates are known: by sqrt(k / found)
+// (points on surfaces: counts grow with the area), clamped to
+// [1.25, FK_GROW_MAX], and FK_GROW_EMPTY * R + 1 with none found.
+#ifndef FK_GROW_MAX
+#define FK_GROW_MAX 2.f
+#endif
+#ifndef FK_GROW_EMPTY
+#define FK_GROW_EMPTY 2
+#endif
+
 __device__ __forceinline__ int next_radius(const CloudInfo &ci, int4 qc,
                                            float4 qr, int Ro, float eff,
                                            int Rcover, int found, int k) {
@@ -746,11 +756,10 @@ __device__ __forceinline__ int next_radius(const CloudInfo &ci, int4 qc,
     }
     Rn = need > (double)Rcover ? Rcover : (int)need;
   } else {
-    long long r2 = 2ll * Ro + 1;
+    long long r2 = (long long)FK_GROW_EMPTY * Ro + 1;
     if (found > 0 && Ro > 0) {
-      const float g = fminf(fmaxf(sqrtf((float)k / (float)found), 1.25f), 2.f);
-      const long long rg = (long long)ceilf(g * (float)Ro);
-      r2 = rg < r2 ? rg : r2;
+      const float g = fminf(fmaxf(sqrtf((float)k / (float)found), 1.25f), FK_GROW_MAX);
+      r2 = (long long)ceilf(g * (float)Ro);
     }
     Rn = r2 > Rcover ? Rcover : (int)r2;
   }
