@@ -80,6 +80,13 @@ def cell_sizes(xyz: torch.Tensor, offsets: torch.Tensor, k, scale=None) -> torch
 
 
 ADAPT_MIN_POINTS = 16384   # smaller clouds keep the first guess
+# One-shot calls (build, one query) skip the feedback rebuilds when every
+# cloud's first guess is within this factor of its candidate target: there
+# the two rebuilds cost more than the query saves (MI355X, build + one K=16
+# query: room 250k 563 -> 460 us, uniform 1M 1392 -> 896 us, the hierarchy's
+# 28k -> 98k stage 445 -> 332 us), while uneven clouds far off target keep
+# them (LiDAR 92k, first guess 13x the target: 546 us, 784 without).
+ONE_SHOT_BAND = 4.0
 
 
 def density_class(mean_candidates, k) -> int:
@@ -92,13 +99,15 @@ def density_class(mean_candidates, k) -> int:
 
 def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
                      target_ratio=None, adapt=True, rounds=None, no_sync=True,
-                     prune=None):
+                     prune=None, skip_band=None):
     """Build ``g`` for exact queries of (xyz, offsets); returns the per-cloud
     cell side.  First guess from the bounding-box density, then (clouds of
     >= ADAPT_MIN_POINTS points) density-feedback rebuilds towards the
     autotuned ``target_ratio * k`` candidates per query.  With ``no_sync``
     the builds use the reserved capacity and read nothing back except the
-    per-cloud first-guess statistic (one read).  Sets ``g.prune_hint``
+    per-cloud first-guess statistic (one read).  With ``skip_band`` b the
+    rebuilds are skipped when every cloud's first-guess candidates per query
+    lie within a factor b of its target.  Sets ``g.prune_hint``
     (FK_QUERY_PRUNE for the queries on this build): ``prune`` if given, else
     the autotuner's choice for the clouds holding most points."""
     n = xyz.shape[0]
@@ -123,15 +132,18 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
         ns = [int(v) for v in ns]
         ks = k.tolist() if isinstance(k, torch.Tensor) else [int(k)] * len(mean)
         dcs = [density_class(mc, kc) for kc, mc in zip(ks, mean)]
-        target_ratio = torch.tensor(
-            [lookup_target_ratio(nc, kc, dc) for nc, kc, dc in zip(ns, ks, dcs)],
-            dtype=torch.float32, device=xyz.device)
+        ratios = [lookup_target_ratio(nc, kc, dc) for nc, kc, dc in zip(ns, ks, dcs)]
+        target_ratio = torch.tensor(ratios, dtype=torch.float32, device=xyz.device)
         def vote(look):  # the choice of the clouds holding most points
             return sum(nc if look(nc, kc, dc) else -nc
                        for nc, kc, dc in zip(ns, ks, dcs)) > 0
         if prune is None:
             g.prune_hint = vote(lookup_prune)
         g.one_lane_hint = vote(lookup_one_lane)
+        if skip_band is not None and all(
+                1.0 / skip_band <= mc / (r * kc) <= skip_band
+                for mc, r, kc in zip(mean, ratios, ks)):
+            return side
     for _ in range(ADAPT_ROUNDS if rounds is None else rounds):
         g.adapt_cell(k, side, inv, target_ratio, CANDIDATE_EXPONENT)
         g.build_xyz(xyz, offsets, inv, no_sync=no_sync)
@@ -157,7 +169,7 @@ class ExactKNN:
         self._cap = (n2, B2, q2)
 
     def build(self, xyz, offsets, k, *, cell=None, scale=None, target_ratio=None,
-              adapt=True, nq=0, no_sync=None, prune=None):
+              adapt=True, nq=0, no_sync=None, prune=None, skip_band=None):
         """Build the support grid; returns the per-cloud cell side.  The
         FK_QUERY_PRUNE choice for queries on it is ``self.grid.prune_hint``.
 
@@ -166,7 +178,9 @@ class ExactKNN:
         box below the reserve (at most ~64 cells per point); a caller's
         ``cell`` may not, so by default such a build sizes its box exactly.
         A no-sync build that overflows is reported by the next synchronising
-        call (``grid.status()``), and queries on it write nothing."""
+        call (``grid.status()``), and queries on it write nothing.
+        ``skip_band``: see ``build_exact_grid`` (one-shot calls pass
+        ONE_SHOT_BAND; a grid that serves many queries keeps the rebuilds)."""
         F.require_device(xyz, "xyz")
         if no_sync is None:
             no_sync = cell is None
@@ -176,7 +190,7 @@ class ExactKNN:
             self._reserve(xyz.shape[0], offsets.shape[0], max(nq, xyz.shape[0]))
         self.side = build_exact_grid(self.grid, xyz, offsets, k, cell=cell, scale=scale,
                                      target_ratio=target_ratio, adapt=adapt,
-                                     no_sync=no_sync, prune=prune)
+                                     no_sync=no_sync, prune=prune, skip_band=skip_band)
         return self.side
 
     @torch.no_grad()
@@ -197,7 +211,7 @@ class ExactKNN:
         try:
             out = self._run(xyz, offsets, k, new_xyz, new_offsets, attached, nq,
                             dict(cell=cell, scale=scale, target_ratio=target_ratio,
-                                 adapt=adapt),
+                                 adapt=adapt, skip_band=ONE_SHOT_BAND),
                             dict(cut_r2=cut_r2, idx_dtype=idx_dtype, pad_mode=pad_mode,
                                  dist_mode=dist_mode, pad_dist=pad_dist), True)
         except F.FlashKNNError as e:
@@ -206,7 +220,7 @@ class ExactKNN:
             # the dense boxes outgrew the reserved capacity: size them exactly
             out = self._run(xyz, offsets, k, new_xyz, new_offsets, attached, nq,
                             dict(cell=cell, scale=scale, target_ratio=target_ratio,
-                                 adapt=adapt),
+                                 adapt=adapt, skip_band=ONE_SHOT_BAND),
                             dict(cut_r2=cut_r2, idx_dtype=idx_dtype, pad_mode=pad_mode,
                                  dist_mode=dist_mode, pad_dist=pad_dist), False)
         if stats:
