@@ -1033,6 +1033,11 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
   const int zlo = zc(zq - Rn), zhi = zc(zq + Rn);
   const int zin_lo = zc(zq - Ro - 1), zin_hi = zc(zq + Ro + 1);
   long long s = lane;  // this lane's next column
+  // (row, offset in row) of column s, stepped by kWave columns at a time
+  // without dividing (64-bit divisions per column dominated sparse shells)
+  const int wyi = wy > 0 ? (int)(wy < INT_MAX ? wy : INT_MAX) : 1;
+  const int step_q = kWave / wyi, step_r = kWave % wyi;
+  int cq = lane / wyi, cr = lane % wyi;
   int p = 0, e = 0;
   // The lane's columns are looked up kShellBatch at a time: their
   // cell-table reads go out together and the non-empty ranges (an inner
@@ -1047,8 +1052,14 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
     for (int j = 0; j < kShellBatch; ++j) {
       const long long sj = s + (long long)j * kWave;
       b[2 * j] = en[2 * j] = b[2 * j + 1] = en[2 * j + 1] = 0;
+      const int dx = x0 + cq, dy = y0 + cr;  // column sj
+      cr += step_r;
+      cq += step_q;
+      if (cr >= wyi) {
+        cr -= wyi;
+        ++cq;
+      }
       if (sj < ncol) {
-        const int dx = x0 + (int)(sj / wy), dy = y0 + (int)(sj % wy);
         const bool inner = dx >= -Ro && dx <= Ro && dy >= -Ro && dy <= Ro;
         column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, zlo, inner ? zin_lo : zhi,
                      b[2 * j], en[2 * j]);
