@@ -65,9 +65,9 @@ def main():
                          "MiB (the known-size reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; the kNN path "
-                          "shards by cloud (see DESIGN.md) but no multi-GPU figure is "
-                          "produced", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "only one GPU is measured: the batched kNN path "
+                          "shards by cloud (each GPU takes whole clouds, no exchange), "
+                          "but no multi-GPU figure is produced", "n_gpus": args.gpus}))
         sys.exit(2)
 
     import numpy as np
