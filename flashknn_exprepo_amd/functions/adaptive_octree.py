@@ -37,6 +37,7 @@ _MAX_COORD = 1 << 16
 _MAX_BATCH = 32768
 _RESERVE_CAP = 1 << 28   # largest dense box a level reserves (cells)
 _SYNC_CELLS = 1 << 21    # boxes above this are built only if some point needs them
+_COLUMN_CELLS_PER_POINT = 64  # separately built levels beyond this take the column layout
 _CHUNK = 1024   # points per chunk of the per-batch minimum
 _SPARE = 4096   # spare rows that absorb the atomics of unused lanes
 
@@ -309,10 +310,15 @@ class AdaptiveNeighborhoodFlashKNN:
         for s in range(first_sep, -1, -1):
             gr = self._level_grid(s)
             cap = box_cells(s) + 64
-            no_sync = cap <= _RESERVE_CAP
+            # sparse boxes (many cells per point) take the column layout: a
+            # synchronising build, but a table of x-y columns instead of the
+            # box's cells to fill and scan (twice: build and attach)
+            columns = cap > _COLUMN_CELLS_PER_POINT * n
+            no_sync = cap <= _RESERVE_CAP and not columns
             if no_sync:  # sized from the host-side spans: no read-back
                 gr.reserve(n, cap, B, n)
-            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync)
+            gr.build_cells(g, batch, pts, num_clouds=B, num_down=s, no_sync=no_sync,
+                           columns=columns)
             counts[s], built[s] = gr.point_candidates(1), gr
             reserved[s] = no_sync
             c = counts[s]
