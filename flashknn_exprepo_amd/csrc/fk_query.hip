@@ -1017,7 +1017,10 @@ struct CoopList {
 
 // Columns of the shell Ro < Chebyshev distance <= Rn, split over the lanes
 // (Ro = -1: the whole block of half-width Rn).
-constexpr int kShellBatch = 8;  // column segments looked up together per lane
+#ifndef FK_SHELL_BATCH
+#define FK_SHELL_BATCH 8
+#endif
+constexpr int kShellBatch = FK_SHELL_BATCH;  // column segments looked up together per lane
 
 template <int P>
 __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &ci,
