@@ -1315,10 +1315,13 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  // After a build that raised an error (an overflowing no-sync build still
+  // writes every cloud's base, past the cell table) no cloud has a tile.
+  const bool ok = scalars[kStatus] == 0;
   for (int c0 = 0; c0 < B; c0 += 1024) {
     const int c = c0 + threadIdx.x;
     int tiles = 0;
-    if (c < B) {
+    if (ok && c < B) {
       const CloudInfo ci = clouds[c];
       const int n = cell_start[ci.base + table_entries(ci)] - cell_start[ci.base];
       tiles = (n + tile - 1) / tile;

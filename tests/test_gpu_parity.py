@@ -502,6 +502,34 @@ def test_batched_graph_replay_matches_eager():
         _ = (torch.randn(1000, device=_dev()) * 2).sum().item()  # eager work between
 
 
+def test_batched_query_after_overflowing_no_sync_build():
+    """A no-sync build whose boxes outgrow the reserve still records every
+    cloud's base, past the cell table: a batched query on it reads no cell
+    table and writes nothing, the next status read reports the overflow, and a
+    synchronising rebuild serves the same handle again."""
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd import synthetic
+    clouds, ks = _batch_case(17)
+    xyz, off = synthetic.pack(clouds)
+    bk = fk.BatchedKNN(_t(xyz), _t(off), ks, exact=True)
+    ei, ed = [t.clone() for t in bk.run()]
+    g = bk.grid
+    g.reserve(len(xyz), 1 << 16, len(ks), len(xyz))
+    fine = torch.full((len(ks),), 1000.0, dtype=torch.float32, device=_dev())  # 1 mm cells
+    g.build_xyz(bk.xyz, bk.offsets, fine, no_sync=True)
+    bk.out_idx.fill_(-7)
+    bk.out_dist.fill_(-7.0)
+    bk.run()
+    with pytest.raises(fk.FlashKNNError) as err:
+        g.status()
+    assert err.value.status == F.FK_ERR_GRID_TOO_LARGE
+    assert (bk.out_idx == -7).all() and (bk.out_dist == -7.0).all()
+    g.build_xyz(bk.xyz, bk.offsets, (1.0 / bk.cell.double()).float())
+    ri, rd = bk.run()
+    g.status()
+    assert torch.equal(ri, ei) and torch.equal(rd, ed)
+
+
 # ------------------------------------------------------------ backward
 
 def test_dist_backward_vs_oracle(oracle):
