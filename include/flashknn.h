@@ -99,7 +99,11 @@ typedef enum fk_status {
 /* build flags */
 #define FK_BUILD_NO_SYNC 1 /* never read sizes back to the host: use the
                               reserved capacity; overflow is reported by
-                              fk_grid_status() (graph-capture safe)         */
+                              fk_grid_status() (graph-capture safe).  Until
+                              the next build, the query kernels of every
+                              entry point (batched included) leave their
+                              output arrays unwritten on a build that raised
+                              an error                                      */
 #define FK_BUILD_COLUMNS 2 /* column layout even when the dense cell box
                               fits (synchronising builds only).  A
                               synchronising build whose dense boxes exceed
