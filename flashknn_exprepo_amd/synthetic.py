@@ -10,7 +10,7 @@
   boxes, range <= 50 m, N(0, 2 cm) range noise, voxel-first subsampled at
   5 cm; density falls off as 1/r^2.
 * ``batch(seed)``: 16 clouds, sizes log-uniform in [2k, 120k], alternating
-  room / LiDAR, per-cloud k in {8, 16, 32}.
+  room / LiDAR generators seeded seed + i, per-cloud k in {8, 16, 32}.
 * ``hierarchy(seed)``: a room of ~100k points subsampled at 12.5/25/50 cm
   (~25k, ~6k, ~1.5k points).
 
@@ -137,18 +137,47 @@ def subsample_to(xyz: np.ndarray, n: int, rng) -> np.ndarray:
     return xyz[sel]
 
 
+# LiDAR sweep sizes after 5 cm voxel subsampling, by azimuth steps (64
+# beams; measured on several seeds, +-5%): a cloud of n points is generated
+# at the smallest step count expected to give about 1.1 n, then subsampled.
+_LIDAR_STEPS = ((64, 3700), (128, 7500), (256, 15000), (512, 29000),
+                (1024, 54000), (1536, 73000), (2048, 88000), (3072, 112000),
+                (4096, 129000), (6144, 150000))
+
+
+def lidar_points(n, seed):
+    """A LiDAR sweep (``lidar``) of seed ``seed`` with exactly ``n`` points:
+    the azimuth resolution is chosen to give at least n points after voxel
+    subsampling (raised until it does), then n of them are kept at random."""
+    i = next((j for j, (_, c) in enumerate(_LIDAR_STEPS) if c >= 1.1 * n),
+             len(_LIDAR_STEPS) - 1)
+    while True:
+        pts = lidar(seed=seed, steps=_LIDAR_STEPS[i][0])
+        if len(pts) >= n or i == len(_LIDAR_STEPS) - 1:
+            break
+        i += 1
+    return subsample_to(pts, n, np.random.default_rng(seed + 10_000))
+
+
+def room_points(n, seed):
+    """A room (``room``) of seed ``seed`` with exactly ``n`` points (calibrated
+    to about 1.05 n after voxel subsampling, then n kept at random)."""
+    pts = room(seed=seed, target=int(n * 1.05) + 16)
+    return subsample_to(pts, n, np.random.default_rng(seed + 10_000))
+
+
 def batch(seed=50, n_clouds=16, lo=2000, hi=120_000):
-    """(list of clouds, list of k): the uneven batched workload."""
+    """(list of clouds, list of k): the uneven batched workload.  Sizes are
+    exp(U(ln lo, ln hi)) and k is drawn from {8, 16, 32} (numpy generator
+    ``seed``); cloud i comes from its own generator seeded ``seed + i``, the
+    room generator for even i and the LiDAR generator for odd i."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(np.log(lo), np.log(hi), n_clouds)).astype(int)
     ks = rng.choice([8, 16, 32], n_clouds)
-    base_room = room(seed=48)
-    base_lidar = lidar(seed=49)
     clouds = []
     for i, n in enumerate(sizes):
-        src = base_room if i % 2 == 0 else base_lidar
-        c = subsample_to(src, int(n), np.random.default_rng(seed + 1 + i))
-        clouds.append(c)
+        make = room_points if i % 2 == 0 else lidar_points
+        clouds.append(make(int(n), seed + i))
     return clouds, [int(k) for k in ks]
 
 

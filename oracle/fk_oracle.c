@@ -236,6 +236,23 @@ int fko_grid_knn(int64_t ns, const float *sxyz, const int64_t *scell,
  * above, out_d is float*.  use_f64 = 1: ranking by the fp64 distance of the
  * fp32 inputs, out_d is double*.  Padding: (pad_idx, +inf).
  */
+/* Insert key k into best[0..*n) (ascending, at most K entries), dropping
+ * the largest when full. */
+static inline void keep_smallest(uint64_t *best, size_t *n, int K, uint64_t k) {
+  size_t m = *n;
+  if (m == (size_t)K) {
+    if (k >= best[K - 1]) return;
+    m = (size_t)K - 1;
+  } else {
+    *n = m + 1;
+  }
+  while (m > 0 && best[m - 1] > k) {
+    best[m] = best[m - 1];
+    --m;
+  }
+  best[m] = k;
+}
+
 int fko_brute_knn(int64_t ns, const float *sxyz, const int64_t *soff,
                   int64_t nq, const float *qxyz, const int64_t *qoff, int nb,
                   int K, float cut_r2, int64_t pad_idx, int use_f64,
@@ -249,18 +266,19 @@ int fko_brute_knn(int64_t ns, const float *sxyz, const int64_t *soff,
     int64_t m = s1 - s0;
 #pragma omp parallel
     {
-      uint64_t *cand = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(m ? m : 1));
+      uint64_t *cand = (uint64_t *)malloc(sizeof(uint64_t) * (size_t)(K > 0 ? K : 1));
       dkey *dc = use_f64 ? (dkey *)malloc(sizeof(dkey) * (size_t)(m ? m : 1)) : NULL;
 #pragma omp for schedule(dynamic, 64)
       for (int64_t q = q0; q < q1; ++q) {
         const float *qp = qxyz + 3 * q;
         size_t n = 0;
         if (!use_f64) {
+          /* the K smallest keys, kept sorted (keys are distinct: the index
+             is in the low bits), equal to the first K of a full sort */
           for (int64_t t = s0; t < s1; ++t) {
             uint64_t k = make_key(d32(sxyz + 3 * t, qp), (uint32_t)t);
-            if (k < cutkey) cand[n++] = k;
+            if (k < cutkey) keep_smallest(cand, &n, K, k);
           }
-          qsort(cand, n, sizeof(uint64_t), cmp_u64);
           float *od = (float *)out_d;
           for (int j = 0; j < K; ++j) {
             if ((size_t)j < n) {

@@ -330,3 +330,91 @@ def test_voxel_first_vs_python_loop(oracle, seed):
         for i, k in enumerate(key.tolist()):
             first.setdefault(k, i)
         assert sorted(first.values()) == got.tolist()
+
+
+# ------------------------------------ adaptive level rule, hand-computed
+# AdaptiveNeighborhoodFlashKNN's level rule
+# (FlashKNN/functions/adaptive_octree.py:327-350): start at the coarsest
+# level; a point is active there if its 27-node count is > max_f * k; at
+# each finer level an active point takes the level if its count there is
+# >= min_f * k, and stays active only if that count is also > max_f * k.
+#
+# Hand case, all points on the x axis (y = z = 0), so a node's 27-node count
+# is the number of same-batch points whose x >> s differs by at most 1:
+#   batch 0: A = 12 points at x 0, B = 3 at x 1, G = 1 at x 3,
+#            D = 1 at x 12, E = 1 at x 15          (max x 15: levels 0..4)
+#   batch 1: H = 8 points at x 7 (shifted to 0 per batch)
+# counts per level s (hand-enumerated):
+#        s=4  s=3  s=2  s=1  s=0
+#   A    18   18   16   16   15     (s=1: A,B in node 0, G in node 1)
+#   B    18   18   16   16   15     (s=0: A at 0, B at 1, G at 3 -> not G)
+#   G    18   18   16   16    1     (s=0: nodes 2..4 hold only G)
+#   D    18   18    2    2    1     (s=2: D,E share node 3; s=1: nodes 6,7)
+#   E    18   18    2    2    1
+#   H     8    8    8    8    8
+_ADAPT_X = [0] * 12 + [1] * 3 + [3, 12, 15] + [7] * 8
+_ADAPT_B = [0] * 18 + [1] * 8
+_ADAPT_COUNTS = {  # level -> per group (A, B, G, D, E, H)
+    4: (18, 18, 18, 18, 18, 8), 3: (18, 18, 18, 18, 18, 8),
+    2: (16, 16, 16, 2, 2, 8), 1: (16, 16, 16, 2, 2, 8), 0: (15, 15, 1, 1, 1, 8)}
+_ADAPT_GROUPS = [12, 3, 1, 1, 1, 8]
+
+
+def _adapt_case():
+    g = np.zeros((len(_ADAPT_X), 3), np.int64)
+    g[:, 0] = _ADAPT_X
+    return g, np.array(_ADAPT_B, np.int64)
+
+
+def _per_group(vals):
+    return np.repeat(np.array(vals, np.int64), _ADAPT_GROUPS)
+
+
+def test_adaptive_hand_counts(oracle):
+    g, b = _adapt_case()
+    gn = g - np.array([[7, 0, 0]]) * (b[:, None] == 1)  # per-batch shift
+    for s, want in _ADAPT_COUNTS.items():
+        assert oracle.point_candidates(gn >> s, b).tolist() == _per_group(want).tolist(), s
+
+
+@pytest.mark.parametrize("k,expect", [
+    # k = 1 (thresholds 2 and 8):
+    #   A, B descend to 0 (15 >= 2 at every level, > 8 down to s = 1);
+    #   G: 16 > 8 at s = 1 but 1 < 2 at s = 0 -> falls back to 1;
+    #   D, E: 2 at s = 2 is >= 2 but not > 8 -> stop at 2 (s = 1 also has 2:
+    #         a rule that kept descending would pick 1);
+    #   H: 8 at the coarsest level is not > 8 -> stays at 4.
+    (1, (0, 0, 1, 2, 2, 4)),
+    # k = 2 (thresholds 4 and 16): A, B, G take s = 2 (16 >= 4) and stop
+    #   there (16 is not > 16); D, E: 2 < 4 at s = 2 -> stay at 3; H stays at 4.
+    (2, (2, 2, 2, 3, 3, 4)),
+])
+def test_adaptive_hand_levels(oracle, k, expect):
+    g, b = _adapt_case()
+    _, d, sel = oracle.adaptive_query(g, b, None, k)
+    assert sel.tolist() == _per_group(expect).tolist()
+    assert np.isfinite(d[:, 0]).all()  # every point finds itself at least
+
+
+# ----------------------------------- back_query cells truncate toward zero
+# back_query takes cells (xyz / query_grid_size).long()
+# (FlashKNN/functions/FlashKnnWrapper.py:480): truncation, so (-size, size)
+# is one double-width cell around 0.  Hand case on the x axis, size 0.1:
+#   p0 x =  0.15 -> cell  1     (support, position 0 in sel)
+#   p1 x = -0.05 -> cell  0     (floor would give -1)
+#   p2 x = -0.25 -> cell -2     (support, position 1; floor: -3)
+#   p3 x = -0.12 -> cell -1     (floor: -2)
+# K = 1 neighbours (position in sel; candidates within one cell):
+#   p0 -> p0;  p1 -> p0 (cells 0 and 1);  p2 -> p2;  p3 -> p2 (cells -1, -2).
+# With floor cells p1 (cell -1) would see neither p0 (cell 1) nor p2
+# (cell -3) and get the padding -1.
+def test_back_query_truncates_negative_cells(oracle):
+    xyz = np.zeros((4, 3), np.float32)
+    xyz[:, 0] = [0.15, -0.05, -0.25, -0.12]
+    sel = np.array([0, 2], np.int64)
+    i1, d1 = oracle.back_query(xyz, sel, 0.1, None, 1)
+    assert i1[:, 0].tolist() == [0, 0, 1, 1]
+    i2, d2 = oracle.back_query(xyz, sel, 0.1, None, 2)
+    assert i2.tolist() == [[0, -1], [0, -1], [1, -1], [1, -1]]
+    assert np.isinf(d2[:, 1]).all() and np.isfinite(d2[:, 0]).all()
+    assert d2[0, 0] == 0 and d2[2, 0] == 0
