@@ -53,24 +53,12 @@ int ok() {
 // (cells + 3), below INT32_MAX; at the cap the four cell tables take 16 GiB.
 constexpr int64_t kHardCellCap = 1ll << 30;
 
-inline hipStream_t S(void *s) { return (hipStream_t)s; }
+// Voxel keys hold the cloud id in bits 48..62 (bit 63 clear marks a used
+// hash slot): at most 2^15 - 1 clouds, the batch limit of the reference's
+// xyz2key (FlashKNN/functions/z_order.py:72-91).
+constexpr int32_t kMaxVoxelClouds = 32767;
 
-template <class T>
-int grow(T *&p, int64_t &cap, int64_t need, const char *what) {
-  if (need <= cap && p) return FK_OK;
-  int64_t n = need + need / 4 + 64;
-  if (p) {
-    FK_HIP(hipFree(p));
-    p = nullptr;
-    cap = 0;
-  }
-  hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)n);
-  if (e != hipSuccess)
-    return fail(FK_ERR_HIP, "hipMalloc of %s (%lld elements) failed: %s", what,
-                (long long)n, hipGetErrorString(e));
-  cap = n;
-  return FK_OK;
-}
+inline hipStream_t S(void *s) { return (hipStream_t)s; }
 
 }  // namespace
 
@@ -136,9 +124,51 @@ struct fk_grid {
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // bumped by every build and by every reallocation of a device table: a
+  // graph captured at another generation holds stale pointers and sizes
+  uint64_t generation = 0, graph_generation = 0;
 };
 
 namespace {
+
+// Grow a device array to at least `need` elements (25% + 64 slack); the old
+// contents are not kept.  A reallocation moves the handle to a new
+// generation (captured graphs hold the old pointers).
+template <class T>
+int grow(fk_grid *g, T *&p, int64_t &cap, int64_t need, const char *what) {
+  if (need <= cap && p) return FK_OK;
+  int64_t n = need + need / 4 + 64;
+  ++g->generation;
+  if (p) {
+    FK_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+  }
+  hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess)
+    return fail(FK_ERR_HIP, "hipMalloc of %s (%lld elements) failed: %s", what,
+                (long long)n, hipGetErrorString(e));
+  cap = n;
+  return FK_OK;
+}
+
+// Exactly `n` elements (reallocated when the capacity differs).
+template <class T>
+int realloc_exact(fk_grid *g, T *&p, int64_t &cap, int64_t n, const char *what) {
+  if (p && cap == n) return FK_OK;
+  ++g->generation;
+  if (p) {
+    FK_HIP(hipFree(p));
+    p = nullptr;
+    cap = 0;
+  }
+  hipError_t e = hipMalloc((void **)&p, sizeof(T) * (size_t)n);
+  if (e != hipSuccess)
+    return fail(FK_ERR_HIP, "hipMalloc of %s (%lld elements) failed: %s", what,
+                (long long)n, hipGetErrorString(e));
+  cap = n;
+  return FK_OK;
+}
 
 int bind_device(fk_grid *g) {
   int dev = 0;
@@ -157,53 +187,62 @@ int bind_device(fk_grid *g) {
 }
 
 int ensure_clouds(fk_grid *g, int64_t B) {
-  FK_CHECK(grow(g->bbox, g->cap_bbox, box_words(B), "bbox"));
-  FK_CHECK(grow(g->clouds, g->cap_clouds, B, "clouds"));
-  FK_CHECK(grow(g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
-  FK_CHECK(grow(g->cloud_acc, g->cap_acc, B, "per-cloud sums"));
+  FK_CHECK(grow(g, g->bbox, g->cap_bbox, box_words(B), "bbox"));
+  FK_CHECK(grow(g, g->clouds, g->cap_clouds, B, "clouds"));
+  FK_CHECK(grow(g, g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
+  FK_CHECK(grow(g, g->cloud_acc, g->cap_acc, B, "per-cloud sums"));
   if (B > g->max_clouds) g->max_clouds = B;
   return FK_OK;
 }
 
 int ensure_points(fk_grid *g, int64_t n) {
   const int64_t m = n < 1 ? 1 : n;
-  FK_CHECK(grow(g->pt_cell, g->cap_points, m, "point cells"));
-  FK_CHECK(grow(g->pt_lin, g->cap_p1, m, "point cell ids"));
-  FK_CHECK(grow(g->pt_rank, g->cap_p2, m, "point ranks"));
-  FK_CHECK(grow(g->srec, g->cap_p3, m, "sorted records"));
-  FK_CHECK(grow(g->scell, g->cap_scell, m, "sorted cells"));
+  FK_CHECK(grow(g, g->pt_cell, g->cap_points, m, "point cells"));
+  FK_CHECK(grow(g, g->pt_lin, g->cap_p1, m, "point cell ids"));
+  FK_CHECK(grow(g, g->pt_rank, g->cap_p2, m, "point ranks"));
+  FK_CHECK(grow(g, g->srec, g->cap_p3, m, "sorted records"));
+  FK_CHECK(grow(g, g->scell, g->cap_scell, m, "sorted cells"));
   return FK_OK;
 }
 
+// The cell tables of a build (counts, starts) and of an attach (query
+// counts, starts) are sized together from one entry capacity c: counts
+// c + 1, starts c + 2 (query tables c + 2 and c + 3, one extra bucket), so
+// that a no-sync build held to cap_cells - 1 entries, or a no-sync attach
+// held to cap_qcells - 1, always fits its start table.
 int ensure_cells(fk_grid *g, int64_t cells) {
-  FK_CHECK(grow(g->counts, g->cap_cells, cells + 1, "cell counts"));
-  FK_CHECK(grow(g->cell_start, g->cap_c2, cells + 2, "cell starts"));
-  FK_CHECK(grow(g->scan_part, g->cap_part, scan_parts_needed(cells + 2), "scan"));
+  if (g->counts && g->cell_start && cells + 1 <= g->cap_cells) return FK_OK;
+  const int64_t c = cells + cells / 4 + 64;
+  FK_CHECK(realloc_exact(g, g->counts, g->cap_cells, c + 1, "cell counts"));
+  FK_CHECK(realloc_exact(g, g->cell_start, g->cap_c2, c + 2, "cell starts"));
+  FK_CHECK(grow(g, g->scan_part, g->cap_part, scan_parts_needed(c + 2), "scan"));
   return FK_OK;
 }
 
 int ensure_qcells(fk_grid *g, int64_t cells) {
-  FK_CHECK(grow(g->qcounts, g->cap_qcells, cells + 2, "query cell counts"));
-  FK_CHECK(grow(g->qstart, g->cap_qc2, cells + 3, "query cell starts"));
-  FK_CHECK(grow(g->scan_part, g->cap_part, scan_parts_needed(cells + 3), "scan"));
+  if (g->qcounts && g->qstart && cells + 2 <= g->cap_qcells) return FK_OK;
+  const int64_t c = cells + cells / 4 + 64;
+  FK_CHECK(realloc_exact(g, g->qcounts, g->cap_qcells, c + 2, "query cell counts"));
+  FK_CHECK(realloc_exact(g, g->qstart, g->cap_qc2, c + 3, "query cell starts"));
+  FK_CHECK(grow(g, g->scan_part, g->cap_part, scan_parts_needed(c + 3), "scan"));
   return FK_OK;
 }
 
 int ensure_queries(fk_grid *g, int64_t nq) {
   const int64_t m = nq < 1 ? 1 : nq;
-  FK_CHECK(grow(g->qrec, g->cap_q, m, "query records"));
-  FK_CHECK(grow(g->qcell, g->cap_q1, m, "query cells"));
-  FK_CHECK(grow(g->q_cell_abs, g->cap_q2, m, "query cells (input order)"));
-  FK_CHECK(grow(g->qpad, g->cap_q3, m, "query pads"));
-  FK_CHECK(grow(g->q_lin, g->cap_q4, m, "query cell ids"));
-  FK_CHECK(grow(g->q_rank, g->cap_q5, m, "query ranks"));
+  FK_CHECK(grow(g, g->qrec, g->cap_q, m, "query records"));
+  FK_CHECK(grow(g, g->qcell, g->cap_q1, m, "query cells"));
+  FK_CHECK(grow(g, g->q_cell_abs, g->cap_q2, m, "query cells (input order)"));
+  FK_CHECK(grow(g, g->qpad, g->cap_q3, m, "query pads"));
+  FK_CHECK(grow(g, g->q_lin, g->cap_q4, m, "query cell ids"));
+  FK_CHECK(grow(g, g->q_rank, g->cap_q5, m, "query ranks"));
   return FK_OK;
 }
 
 int ensure_pending(fk_grid *g, int64_t nq, int k) {
   const int64_t m = nq < 1 ? 1 : nq;
-  FK_CHECK(grow(g->pend_q, g->cap_pend, m, "pending queries"));
-  FK_CHECK(grow(g->pend_keys, g->cap_pkeys, m * k, "pending keys"));
+  FK_CHECK(grow(g, g->pend_q, g->cap_pend, m, "pending queries"));
+  FK_CHECK(grow(g, g->pend_keys, g->cap_pkeys, m * k, "pending keys"));
   return FK_OK;
 }
 
@@ -309,11 +348,11 @@ int finish_build(fk_grid *g, const float *xyz, const int64_t *grid, int64_t n,
     size_t tmp = 0;
     FK_HIP(column_sort_tmp_bytes(n, key_bits, &tmp));
     const int64_t m = n < 1 ? 1 : n;
-    FK_CHECK(grow(g->ckeys0, g->cap_ck0, m, "column keys"));
-    FK_CHECK(grow(g->ckeys1, g->cap_ck1, m, "column keys"));
-    FK_CHECK(grow(g->cperm0, g->cap_cp0, m, "column order"));
-    FK_CHECK(grow(g->cperm1, g->cap_cp1, m, "column order"));
-    FK_CHECK(grow(g->sort_tmp, g->cap_tmp, (int64_t)tmp + 1, "column sort"));
+    FK_CHECK(grow(g, g->ckeys0, g->cap_ck0, m, "column keys"));
+    FK_CHECK(grow(g, g->ckeys1, g->cap_ck1, m, "column keys"));
+    FK_CHECK(grow(g, g->cperm0, g->cap_cp0, m, "column order"));
+    FK_CHECK(grow(g, g->cperm1, g->cap_cp1, m, "column order"));
+    FK_CHECK(grow(g, g->sort_tmp, g->cap_tmp, (int64_t)tmp + 1, "column sort"));
     bb = build_buffers(g);
     FK_HIP(launch_sort_columns(bb, xyz, grid, n, hint, depth_bits, key_bits, s));
   } else {
@@ -321,6 +360,7 @@ int finish_build(fk_grid *g, const float *xyz, const int64_t *grid, int64_t n,
   }
   g->build_cap = hint;
   g->built = 1;
+  ++g->generation;  // new tables and sizes: a captured step is stale
   g->n_points = n;
   g->n_clouds = B;
   g->n_queries = -1;
@@ -851,11 +891,15 @@ int fk_batched_capture(fk_grid *g, const fk_query_params *p,
   g->graph = graph;
   FK_HIP(hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0));
   g->batched_nq = a.exact ? a.nq : 0;
+  g->graph_generation = g->generation;
   return ok();
 }
 
 int fk_graph_replay(fk_grid *g, void *stream) {
   if (!g || !g->gexec) return fail(FK_ERR_STATE, "fk_graph_replay: nothing captured");
+  if (g->generation != g->graph_generation)
+    return fail(FK_ERR_STATE, "fk_graph_replay: the grid was rebuilt or its tables "
+                              "reallocated since the capture; capture again");
   FK_HIP(hipGraphLaunch(g->gexec, S(stream)));
   g->stat_set = kBatchedSet;
   g->stat_nq = g->batched_nq;
@@ -902,10 +946,11 @@ int64_t fk_voxel_workspace_bytes(int64_t n) {
 int fk_voxel_first(const float *xyz, const int32_t *offsets, int32_t num_clouds,
                    int64_t n, float voxel_size, void *workspace,
                    int64_t *out_idx, int64_t *out_count, void *stream) {
-  if (n < 0 || n > INT32_MAX / 2 || num_clouds < 1 ||
+  if (n < 0 || n > INT32_MAX / 2 || num_clouds < 1 || num_clouds > kMaxVoxelClouds ||
       (!offsets && num_clouds != 1) || (n > 0 && (!xyz || !out_idx)) ||
       !workspace || !out_count)
-    return fail(FK_ERR_INVALID_ARG, "fk_voxel_first: bad arguments");
+    return fail(FK_ERR_INVALID_ARG, "fk_voxel_first: bad arguments (num_clouds in "
+                                    "[1, 32767], the reference's batch limit)");
   if (!(voxel_size > 0.f) || !std::isfinite(voxel_size))
     return fail(FK_ERR_INVALID_ARG,
                 "fk_voxel_first: voxel_size must be finite and > 0");

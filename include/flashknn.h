@@ -277,7 +277,12 @@ int fk_batched_query(fk_grid *grid, const fk_query_params *params,
                      const int32_t *k_per_cloud, const int64_t *out_offsets,
                      void *out_idx, float *out_dist, void *stream);
 /* Capture fk_batched_query into a hipGraph owned by the handle (the
- * arguments are frozen); fk_graph_replay launches it on `stream`. */
+ * arguments are frozen); fk_graph_replay launches it on `stream`.  The
+ * graph holds the handle's table pointers and the sizes of the build it
+ * was captured on: after any later build on the handle, or any call that
+ * reallocated its tables (a larger fk_grid_reserve, fk_query or
+ * fk_batched_query needing more pending-query room), fk_graph_replay
+ * launches nothing and returns FK_ERR_STATE; capture again. */
 int fk_batched_capture(fk_grid *grid, const fk_query_params *params,
                        const int32_t *k_per_cloud, const int64_t *out_offsets,
                        void *out_idx, float *out_dist, void *stream);
@@ -339,7 +344,9 @@ int fk_idw_interp_backward(const float *q, int64_t nq, const float *p, int64_t n
  * indices ascending to out_idx (int64[n], device) and their count to
  * *out_count (int64, device; -status on a device-side error: a coordinate
  * outside |v| < 2^31 or not finite, a point past offsets[num_clouds-1]).
- * offsets: int32[num_clouds] cumulative ends (device) or NULL for one cloud.
+ * offsets: int32[num_clouds] cumulative ends (device) or NULL for one cloud;
+ * num_clouds <= 32767 (the reference's batch limit; the key holds the cloud
+ * in 15 bits), else FK_ERR_INVALID_ARG.
  * No host synchronisation; workspace (device) must hold
  * fk_voxel_workspace_bytes(n) bytes. */
 int64_t fk_voxel_workspace_bytes(int64_t n);

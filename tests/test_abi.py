@@ -83,3 +83,24 @@ def test_header_constants_match_python():
         if name == "FK_ABI_VERSION":
             continue
         assert getattr(_lib, name) == int(v), name
+
+
+def test_voxel_first_refuses_too_many_clouds():
+    """Voxel keys hold the cloud id in 15 bits: 32768 clouds are refused
+    before anything is launched (the pointers are never touched)."""
+    from flashknn_exprepo_amd import _lib
+    L = _lib.load()
+    fake = ctypes.c_void_p(16)
+    st = L.fk_voxel_first(fake, fake, 32768, 10, ctypes.c_float(0.1), fake, fake, fake,
+                          None)
+    assert st == _lib.FK_ERR_INVALID_ARG
+    assert b"32767" in L.fk_last_error()
+
+
+def test_graph_replay_without_capture_is_refused():
+    from flashknn_exprepo_amd import _lib
+    L = _lib.load()
+    h = ctypes.c_void_p()
+    assert L.fk_grid_create(ctypes.byref(h)) == 0
+    assert L.fk_graph_replay(h, None) == _lib.FK_ERR_STATE
+    assert L.fk_grid_destroy(h) == 0
