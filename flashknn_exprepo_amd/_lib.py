@@ -41,6 +41,7 @@ FK_BUILD_NO_SYNC = 1
 FK_BUILD_COLUMNS = 2
 FK_QUERY_PRUNE = 1
 FK_QUERY_ONE_LANE = 2
+FK_VOXEL_NOALLOC_MAX_POINTS = 1048576
 
 
 class FlashKNNError(RuntimeError):
@@ -124,6 +125,7 @@ SIGNATURES = [
                                               ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("fk_voxel_workspace_bytes", ctypes.c_int64, [_i64]),
     ("fk_voxel_first", ctypes.c_int, [_vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp, _vp, _vp]),
+    ("fk_voxel_first_noalloc", ctypes.c_int, [_vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp, _vp]),
     ("fk_backward_workspace_bytes", ctypes.c_int64, [_i64, _i64, _i32]),
     ("fk_dist_backward", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _vp, _vp, _vp, _vp]),
     ("fk_idw_forward", ctypes.c_int, [_vp, _i64, _i32, _vp, _vp, _i64, _i32, _vp, _vp]),

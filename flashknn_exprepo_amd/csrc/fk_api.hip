@@ -959,6 +959,26 @@ int fk_voxel_first(const float *xyz, const int32_t *offsets, int32_t num_clouds,
   return ok();
 }
 
+int fk_voxel_first_noalloc(const float *xyz, const int32_t *offsets,
+                           int32_t num_clouds, int64_t n, float voxel_size,
+                           int64_t *out_idx, int64_t *out_count, void *stream) {
+  if (n < 0 || num_clouds < 1 || num_clouds > kMaxVoxelClouds ||
+      (!offsets && num_clouds != 1) || (n > 0 && (!xyz || !out_idx)) || !out_count)
+    return fail(FK_ERR_INVALID_ARG, "fk_voxel_first_noalloc: bad arguments (num_clouds "
+                                    "in [1, 32767])");
+  if (n > FK_VOXEL_NOALLOC_MAX_POINTS)
+    return fail(FK_ERR_CAPACITY, "fk_voxel_first_noalloc: %lld points is above the "
+                                 "static workspace (%d points); use fk_voxel_first "
+                                 "with a workspace", (long long)n,
+                FK_VOXEL_NOALLOC_MAX_POINTS);
+  if (!(voxel_size > 0.f) || !std::isfinite(voxel_size))
+    return fail(FK_ERR_INVALID_ARG,
+                "fk_voxel_first_noalloc: voxel_size must be finite and > 0");
+  FK_HIP(launch_voxel_first_noalloc(xyz, offsets, num_clouds, n, voxel_size, out_idx,
+                                    out_count, S(stream)));
+  return ok();
+}
+
 int64_t fk_backward_workspace_bytes(int64_t nq, int64_t ns, int32_t k) {
   if (nq < 0 || ns < 0 || k < 1) return -1;
   return backward_workspace_bytes(nq, ns, k);

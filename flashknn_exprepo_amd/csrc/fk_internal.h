@@ -306,6 +306,12 @@ hipError_t launch_voxel_first(const float *xyz, const int32_t *offsets,
                               int32_t B, int64_t n, float size, void *ws,
                               int64_t *out_idx, int64_t *out_count,
                               hipStream_t s);
+// The same over the library's static per-device workspace (n at most
+// FK_VOXEL_NOALLOC_MAX_POINTS; hipErrorInvalidValue above).  Allocates nothing.
+hipError_t launch_voxel_first_noalloc(const float *xyz, const int32_t *offsets,
+                                      int32_t B, int64_t n, float size,
+                                      int64_t *out_idx, int64_t *out_count,
+                                      hipStream_t s);
 
 int64_t backward_workspace_bytes(int64_t nq, int64_t ns, int32_t k);
 hipError_t launch_dist_backward(const float *q, int64_t nq, const float *p,

@@ -19,6 +19,7 @@
 #include "fk_internal.h"
 
 #include <climits>
+#include <mutex>
 
 namespace fk {
 
@@ -163,6 +164,32 @@ __global__ void k_vox_compact(const int32_t *flags, const int32_t *pos,
 
 }  // namespace
 
+// ------------------------------------------------ allocation-free variant
+//
+// fk_voxel_first_noalloc keeps its workspace in one file-scope device array
+// (each GPU holds its own copy, at its own address, until the process ends)
+// instead of a caller-provided buffer: for callers that cannot allocate
+// (graph capture, serving loops).  Sized for kNoAllocMaxPoints points; the
+// host side takes the array's device address per device with
+// hipGetSymbolAddress and never passes the array's name to a kernel.
+constexpr int64_t kNoAllocMaxPoints = FK_VOXEL_NOALLOC_MAX_POINTS;
+
+constexpr int64_t const_slots(int64_t n) {
+  int64_t s = 64;
+  while (s < 2 * n) s <<= 1;
+  return s;
+}
+constexpr int64_t const_align(int64_t v) { return (v + 255) & ~int64_t(255); }
+// the voxel workspace of kNoAllocMaxPoints points (the layout of carve(),
+// scan partials bounded by n / 4096 + 3)
+constexpr int64_t kScratchBytes =
+    256 + const_align(8 * const_slots(kNoAllocMaxPoints)) +
+    const_align(4 * const_slots(kNoAllocMaxPoints)) +
+    2 * const_align(4 * kNoAllocMaxPoints) + const_align(4 * (kNoAllocMaxPoints + 1)) +
+    const_align(4 * (kNoAllocMaxPoints / 4096 + 3));
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 int64_t voxel_workspace_bytes(int64_t n) {
   const int64_t S = table_slots(n);
   return 256 + align_up(8 * S) + align_up(4 * S) + 2 * align_up(4 * n) +
@@ -192,6 +219,41 @@ hipError_t launch_voxel_first(const float *xyz, const int32_t *offsets,
   hipLaunchKernelGGL(k_vox_compact, dim3(nblocks(n)), dim3(kBlock), 0, s,
                      w.flags, w.pos, (int32_t)n, w.status, out_idx, out_count);
   return hipGetLastError();
+}
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_addr[kMaxDevices] = {};  // device address of `scratch` per device
+}  // namespace
+
+hipError_t launch_voxel_first_noalloc(const float *xyz, const int32_t *offsets,
+                                      int32_t B, int64_t n, float size,
+                                      int64_t *out_idx, int64_t *out_count,
+                                      hipStream_t s) {
+  if (n > kNoAllocMaxPoints || voxel_workspace_bytes(n) > kScratchBytes)
+    return hipErrorInvalidValue;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  void *ws = nullptr;
+  {
+    // first call on this device (it is current): look the array up once
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+      void *p = nullptr;
+      e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+      if (e != hipSuccess) return e;
+      g_scratch_addr[dev] = p;
+    }
+    ws = g_scratch_addr[dev];
+  }
+  // every workspace word the kernels read is written first in the same call
+  // (hash table and status by k_vox_init, flags, scan partials and
+  // positions by their producers), so the previous call's contents are
+  // harmless
+  return launch_voxel_first(xyz, offsets, B, n, size, ws, out_idx, out_count, s);
 }
 
 }  // namespace fk

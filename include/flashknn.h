@@ -354,6 +354,20 @@ int fk_voxel_first(const float *xyz, const int32_t *offsets, int32_t num_clouds,
                    int64_t n, float voxel_size, void *workspace,
                    int64_t *out_idx, int64_t *out_count, void *stream);
 
+/* fk_voxel_first without a workspace argument and without any allocation:
+ * no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The workspace is a static array of the library, one
+ * per GPU (36 MiB, held until the process ends), sized for
+ * n <= FK_VOXEL_NOALLOC_MAX_POINTS; larger n returns FK_ERR_CAPACITY and
+ * launches nothing.  Results are those of fk_voxel_first.  Every call on
+ * one GPU uses that GPU's one array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define FK_VOXEL_NOALLOC_MAX_POINTS 1048576
+int fk_voxel_first_noalloc(const float *xyz, const int32_t *offsets,
+                           int32_t num_clouds, int64_t n, float voxel_size,
+                           int64_t *out_idx, int64_t *out_count, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
