@@ -41,6 +41,7 @@ FK_BUILD_NO_SYNC = 1
 FK_BUILD_COLUMNS = 2
 FK_QUERY_PRUNE = 1
 FK_QUERY_ONE_LANE = 2
+FK_QUERY_UNIFORM = 4
 FK_VOXEL_NOALLOC_MAX_POINTS = 1048576
 
 

@@ -53,6 +53,11 @@ int ok() {
 // (cells + 3), below INT32_MAX; at the cap the four cell tables take 16 GiB.
 constexpr int64_t kHardCellCap = 1ll << 30;
 
+// Balanced first pass: self-query launches from this size (below, groups
+// widen to fill the GPU instead) up to kPlanMaxTilesHost super-tiles.
+constexpr int32_t kPlanMinQueries = 65536;
+constexpr int64_t kPlanMaxTilesHost = 8192;
+
 // Voxel keys hold the cloud id in bits 48..62 (bit 63 clear marks a used
 // hash slot): at most 2^15 - 1 clouds, the batch limit of the reference's
 // xyz2key (FlashKNN/functions/z_order.py:72-91).
@@ -124,6 +129,14 @@ struct fk_grid {
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // balanced first pass of self queries: per-wave plan, super-tile costs,
+  // and what the plan was computed for
+  int2 *plan = nullptr;
+  int64_t cap_plan = 0;
+  int32_t *tile_cost = nullptr;
+  int64_t cap_tcost = 0;
+  uint64_t plan_generation = ~0ull;
+  int32_t plan_kc = 0, plan_R = 0, plan_capacity = 0;
   // bumped by every build and by every reallocation of a device table: a
   // graph captured at another generation holds stale pointers and sizes
   uint64_t generation = 0, graph_generation = 0;
@@ -433,7 +446,7 @@ int fk_grid_destroy(fk_grid *g) {
                   g->cell_start, g->qcounts, g->qstart, g->scan_part, g->qrec,
                   g->qcell, g->q_cell_abs, g->qpad, g->q_lin, g->q_rank,
                   g->pend_q, g->pend_keys, g->ckeys0, g->ckeys1, g->cperm0,
-                  g->cperm1, g->sort_tmp};
+                  g->cperm1, g->sort_tmp, g->plan, g->tile_cost};
   for (void *p : bufs)
     if (p) note(hipFree(p));
   if (g->h_pinned) note(hipHostFree(g->h_pinned));
@@ -664,6 +677,37 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   return FK_OK;
 }
 
+// The balanced first pass (launch_plan) for self queries of K classes
+// 8-32 on launches large enough not to widen their lane groups and small
+// enough for one resident round (at most kPlanTile queries per resident
+// wave); the plan is computed once per build and kept.
+static int attach_plan(fk_grid *g, const fk_query_params *p, QueryArgs &a,
+                       hipStream_t s) {
+  a.plan = nullptr;
+  a.plan_waves = 0;
+  if (p->query_set != FK_QUERIES_SELF || a.k > 32 || a.one_lane ||
+      (p->flags & FK_QUERY_UNIFORM) || !(a.passes & 2) || a.nq < kPlanMinQueries)
+    return FK_OK;
+  const int kc = kclass_of(a.k);
+  int32_t cap = 0;
+  FK_HIP(planned_capacity(kc, &cap));
+  const int64_t ntiles = ((int64_t)a.nq + kPlanTile - 1) / kPlanTile;
+  if (ntiles > cap || ntiles > kPlanMaxTilesHost) return FK_OK;
+  FK_CHECK(grow(g, g->plan, g->cap_plan, cap, "first-pass plan"));
+  FK_CHECK(grow(g, g->tile_cost, g->cap_tcost, ntiles, "super-tile costs"));
+  if (g->plan_generation != g->generation || g->plan_kc != kc || g->plan_R != a.R ||
+      g->plan_capacity != cap) {
+    FK_HIP(launch_plan(a, cap, g->tile_cost, g->plan, s));
+    g->plan_generation = g->generation;
+    g->plan_kc = kc;
+    g->plan_R = a.R;
+    g->plan_capacity = cap;
+  }
+  a.plan = g->plan;
+  a.plan_waves = cap;
+  return FK_OK;
+}
+
 int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
              float *out_dist, void *stream) {
   FK_CHECK(check_stream_device(g));
@@ -684,6 +728,7 @@ int fk_query(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.qcnt = g->scalars + kCounters + kCntStride * set;
   a.qcnt_next = first && a.nq > 0 ? g->scalars + kCounters + kCntStride * (set ^ 1) : nullptr;
   a.passes = (a.passes & ~1) | (reset ? 1 : 0);
+  FK_CHECK(attach_plan(g, p, a, S(stream)));
   FK_HIP(launch_query(a, S(stream)));
   g->qset = set;
   g->stat_set = set;

@@ -250,9 +250,32 @@ struct QueryArgs {
   int32_t num_clouds;
   int32_t num_items;     // work items (cloud, 64-query tile)
   const int32_t *item_cloud;   // unused when 0 items
+  // balanced first pass (self queries): per-wave (first query, lanes per
+  // query) of a launch of plan_waves waves; null: uniform 32-query tiles
+  const int2 *plan;
+  int32_t plan_waves;
 };
 
 hipError_t launch_query(const QueryArgs &a, hipStream_t s);
+
+// Balanced first pass.  The queries are cut into super-tiles of
+// kPlanTile consecutive (cell-sorted) queries; a super-tile whose largest
+// block candidate count is c runs on G = clamp(pow2ceil(c / L), 1, 8) waves
+// (G lanes per query, kWave / G queries per wave), with L the smallest
+// per-lane budget for which the waves fit `capacity`, heaviest super-tiles
+// first.  A uniform launch is set by its slowest tiles (dense regions hold
+// several times the median candidate count and each lane scans its records
+// one round after another); the plan gives those more lanes and light ones
+// fewer.  Results do not depend on the plan.
+constexpr int kPlanTile = 64;
+// Resident waves of the planned first pass of K class kc on this device.
+hipError_t planned_capacity(int kc, int32_t *waves);
+// Writes plan[0, capacity) (first query, lanes); tile_cost: int32[ceil(nq /
+// kPlanTile)] scratch.  Needs nq <= kPlanTile * capacity.
+hipError_t launch_plan(const QueryArgs &a, int32_t capacity, int32_t *tile_cost,
+                       int2 *plan, hipStream_t s);
+hipError_t launch_plan_columns(const QueryArgs &a, int32_t capacity,
+                               int32_t *tile_cost, int2 *plan, hipStream_t s);
 hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s);
 
 int kclass_of(int k);  // 8, 16, 32, 64 or 128

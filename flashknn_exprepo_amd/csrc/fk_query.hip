@@ -278,11 +278,13 @@ struct TopK {
     refresh_thr();
   }
 
+  // Append key if ok and below the threshold.  The store is unconditional
+  // (slot cnt is free: a rejected key is overwritten by the next append), so
+  // the append is a store and an add instead of a masked branch; callers
+  // guarantee cnt < buffer_slots<KC>() at every call.
   __device__ __forceinline__ void push(uint64_t key, bool ok) {
-    if (ok && key < thr) {
-      lb[cnt * kWave] = key;
-      ++cnt;
-    }
+    lb[cnt * kWave] = key;
+    cnt += (ok && key < thr) ? 1 : 0;
   }
 
   __device__ __forceinline__ uint64_t kth() const { return top[KC - 1]; }
@@ -369,7 +371,7 @@ __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, int sub,
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const bool ok = s + u < len;
-        r[u] = srec[ok ? first + (s + u) * G : 0];
+        r[u] = srec[ok ? (uint32_t)(first + (s + u) * G) : 0u];
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
@@ -549,14 +551,14 @@ struct ColumnStream {
       l = (r.y - r.x + G - 1) / G;
     }
   }
-  __device__ __forceinline__ int addr(int u, bool &ok) const {
-    if (u < lc) {
-      ok = true;
-      return pc + u * G;
-    }
-    const int v = u - lc;
-    ok = v < ln;
-    return pn + v * G;
+  // Record of slot u of this round: the current range's u-th, or the
+  // (u - lc)-th of the following one; index 0 for a slot past both (its
+  // load is discarded).  One select per slot: pn + (u - lc) G is
+  // (pn - lc G) + u G.
+  __device__ __forceinline__ uint32_t addr(int u, bool &ok) const {
+    ok = u < lc + ln;
+    const int base = u < lc ? pc : pn - lc * G;
+    return ok ? (uint32_t)(base + u * G) : 0u;
   }
 
   // After a round of kSlots records: the current range continues, or the
@@ -591,8 +593,7 @@ __device__ __forceinline__ void scan_stream(TopK<KC> &t, ColumnStream<NC, G> &cs
       bool ok[kSlots];
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
-        const int ad = cs.addr(u, ok[u]);
-        r[u] = srec[ok[u] ? ad : 0];
+        r[u] = srec[cs.addr(u, ok[u])];
       }
 #pragma unroll
       for (int u = 0; u < kSlots; ++u) {
@@ -908,6 +909,37 @@ k_query(QueryArgs a) {
   if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
   run_tile<KC, G>(a, q0, a.nq, sbuf + w * BS * kWave + lane,
                   stab + w * stream_entries<KC>() * kWave + lane);
+}
+
+// The balanced first pass: wave w runs plan[w] = (first query, G) with G
+// lanes per query (see launch_plan).  One kernel serves every G, so a
+// launch mixes wide groups for the dense super-tiles with narrow ones for
+// the sparse.
+template <int KC>
+__global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
+k_query_planned(QueryArgs a) {
+  constexpr int WPB = waves_per_block<KC>();
+  constexpr int BS = buffer_slots<KC>();
+  __shared__ uint64_t sbuf[WPB * BS * kWave];
+  __shared__ uint64_t stab[WPB * stream_entries<KC>() * kWave];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
+  const int wid = blockIdx.x * WPB + w;
+  if (wid >= a.plan_waves || a.scalars[kStatus] != 0) return;
+  const int2 d = a.plan[wid];
+  const int q0 = __builtin_amdgcn_readfirstlane(d.x);
+  const int g = __builtin_amdgcn_readfirstlane(d.y);
+  if (q0 >= a.nq) return;
+  uint64_t *lb = sbuf + w * BS * kWave + lane;
+  uint64_t *lt = stab + w * stream_entries<KC>() * kWave + lane;
+  const int qend = min(a.nq, q0 + kWave / g);
+  switch (g) {
+    case 1: run_tile<KC, 1>(a, q0, qend, lb, lt); break;
+    case 2: run_tile<KC, 2>(a, q0, qend, lb, lt); break;
+    case 4: run_tile<KC, 4>(a, q0, qend, lb, lt); break;
+    default: run_tile<KC, 8>(a, q0, qend, lb, lt); break;
+  }
 }
 
 // ------------------------------------------------- cooperative expansion
@@ -1391,6 +1423,136 @@ k_candidate_count(const QueryArgs a, unsigned long long *total) {
   }
 }
 
+// Largest block candidate count of every super-tile of kPlanTile = kWave
+// consecutive queries (one wave per super-tile).
+__global__ void __launch_bounds__(256) k_tile_cost(const QueryArgs a, int32_t *tile_cost) {
+  const int qi = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = 0;
+  if (qi < a.nq && a.scalars[kStatus] == 0) {
+    int cloud;
+    const unsigned long long v = query_candidates(a, qi, cloud);
+    c = v < (unsigned long long)INT_MAX ? (int)v : INT_MAX;
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o, kWave));
+  if ((threadIdx.x & (kWave - 1)) == 0 && qi < a.nq) tile_cost[qi / kPlanTile] = c;
+}
+
+constexpr int kPlanMaxTiles = 8192;  // super-tiles cached in LDS by k_plan
+
+constexpr int kPlanSearch = 7;       // bisection steps over the 2^7 budgets
+
+// One lane per query (64 queries per wave) measured about 1.4x slower per
+// candidate than two lanes per query (a lane's buffer fills from one query's
+// stream, with no threshold shared in the group): a super-tile takes one
+// lane only if its cost with that factor fits the budget.
+#ifndef FK_PLAN_ONE_LANE_COST
+#define FK_PLAN_ONE_LANE_COST 2.0f
+#endif
+constexpr int kPlanGmin = 1;
+
+// Lanes per query of a super-tile whose largest count is `cost` under the
+// per-lane budget L: 1 if cost * FK_PLAN_ONE_LANE_COST <= L, else the
+// smallest power of two in [2, 8] with cost <= G * L.
+__device__ __forceinline__ int plan_lanes(int cost, float L) {
+  if ((float)cost * FK_PLAN_ONE_LANE_COST <= L) return 1;
+  int g = 2;
+  while (g < 8 && (float)cost > (float)g * L) g <<= 1;
+  return g;
+}
+
+// Budget level j (0..127): L = 4 * 2^(j / 8) candidates per lane.
+__device__ __forceinline__ float plan_budget(int j) { return 4.f * exp2f(0.125f * (float)j); }
+
+// One block: the smallest budget whose waves fit `capacity`, then the
+// per-wave plan, super-tiles with more lanes first (they are the longest
+// waves: started first, they finish with the rest).  Unused plan entries
+// get (nq, 1) and exit at once.
+__global__ void __launch_bounds__(1024)
+k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
+       const int32_t *scalars, int2 *plan) {
+  __shared__ int cost[kPlanMaxTiles];
+  __shared__ int red[16];
+  __shared__ int cls_n[4], carry[4];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const bool ok = scalars[kStatus] == 0;
+  if (!ok) {
+    for (int i = t; i < capacity; i += 1024) plan[i] = make_int2(nq, 1);
+    return;
+  }
+  for (int i = t; i < ntiles; i += 1024) cost[i] = tile_cost[i];
+  __syncthreads();
+  auto block_sum = [&](int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (lane == 0) red[w] = v;
+    __syncthreads();
+    int s = 0;
+    for (int j = 0; j < 16; ++j) s += red[j];
+    __syncthreads();
+    return s;
+  };
+  // waves(L) does not grow with L: bisect the budget levels
+  int lo = 0, hi = (1 << kPlanSearch) - 1;  // hi always fits (8192 per lane)
+  for (int it = 0; it < kPlanSearch; ++it) {
+    const int mid = (lo + hi) >> 1;
+    const float L = plan_budget(mid);
+    int v = 0;
+    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L);
+    if (block_sum(v) <= capacity) hi = mid; else lo = mid + 1;
+  }
+  // the last level puts every super-tile on kPlanGmin lanes per query
+  // (waves = kPlanGmin * super-tiles <= capacity, which the host checks)
+  const float L = hi == (1 << kPlanSearch) - 1 ? INFINITY : plan_budget(hi);
+  // per class (G = 8, 4, 2, 1): super-tile count, then first wave of the class
+  for (int c = 0; c < 4; ++c) {
+    int v = 0;
+    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L) == (8 >> c);
+    const int n = block_sum(v);
+    if (t == 0) cls_n[c] = n;
+  }
+  if (t == 0) {
+    int base = 0;
+    for (int c = 0; c < 4; ++c) {
+      const int n = cls_n[c];
+      carry[c] = base;  // running first wave of the next super-tile of class c
+      base += n * (8 >> c);
+    }
+  }
+  __syncthreads();
+  __shared__ int wcnt[4][16];
+  for (int i0 = 0; i0 < ntiles; i0 += 1024) {
+    const int i = i0 + t;
+    const int g = i < ntiles ? plan_lanes(cost[i], L) : 0;
+    const int c = g == 8 ? 0 : (g == 4 ? 1 : (g == 2 ? 2 : 3));
+    int pos = 0;
+    for (int cc = 0; cc < 4; ++cc) {
+      const uint64_t m = __ballot(g != 0 && c == cc);
+      if (lane == 0) wcnt[cc][w] = __popcll(m);
+      if (g != 0 && c == cc) pos = __popcll(m & ((1ull << lane) - 1));
+    }
+    __syncthreads();
+    if (g != 0) {
+      int before = 0;  // super-tiles of class c ahead of this one in the chunk
+      for (int j = 0; j < w; ++j) before += wcnt[c][j];
+      const int first = carry[c] + (before + pos) * g;
+      for (int p = 0; p < g; ++p)
+        if (first + p < capacity)  // always (the waves fit by construction)
+          plan[first + p] = make_int2(i * kPlanTile + p * (kPlanTile / g), g);
+    }
+    __syncthreads();
+    if (t < 4) {
+      int tot = 0;
+      for (int j = 0; j < 16; ++j) tot += wcnt[t][j];
+      carry[t] += tot * (8 >> t);
+    }
+    __syncthreads();
+  }
+  int used = 0;
+  for (int c = 0; c < 4; ++c) used += cls_n[c] * (8 >> c);
+  for (int i = used + t; i < capacity; i += 1024) plan[i] = make_int2(nq, 1);
+}
+
 // Candidate count |C(q)| of every query, written to its output row.
 __global__ void k_point_candidates(const QueryArgs a, int32_t *out) {
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1541,6 +1703,14 @@ template <int KC>
 hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
   constexpr int G = lanes_per_query<KC>();
   if (a.passes & 2) {
+    if constexpr (KC <= 32) {
+      if (a.plan) {
+        constexpr int WPB = waves_per_block<KC>();
+        hipLaunchKernelGGL(k_query_planned<KC>, dim3(a.plan_waves / WPB), dim3(64 * WPB),
+                           0, s, a);
+        goto expand;
+      }
+    }
     if (G > 1 && a.one_lane)
       launch_first<KC, 1>(a, s);
     else if constexpr (G > 1)
@@ -1548,6 +1718,7 @@ hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
     else
       launch_first<KC, G>(a, s);
   }
+expand:
   if (a.exact && (a.passes & 4))
     hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)),
                        dim3(64 * kCoopWaves), 0, s, a);
@@ -1604,6 +1775,60 @@ hipError_t FK_LAYOUT(launch_query)(const QueryArgs &a, hipStream_t s) {
     default: return launch_plain<64>(a, s);
   }
 }
+
+hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
+                                  int32_t *tile_cost, int2 *plan, hipStream_t s) {
+  FK_FORWARD(launch_plan_columns(a, capacity, tile_cost, plan, s));
+  const int64_t ntiles = ((int64_t)a.nq + kPlanTile - 1) / kPlanTile;
+  if (a.nq <= 0 || ntiles * kPlanGmin > capacity || ntiles > kPlanMaxTiles)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_tile_cost, dim3((a.nq + 255) / 256), dim3(256), 0, s, a, tile_cost);
+  hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, tile_cost, (int32_t)ntiles, a.nq,
+                     capacity, a.scalars, plan);
+  return hipGetLastError();
+}
+
+#if !FK_COLUMNS
+// Waves a plan may use beyond one resident round, percent (measured: any
+// over-subscription was slower than fitting the round).
+#ifndef FK_PLAN_OVER
+#define FK_PLAN_OVER 0
+#endif
+constexpr int kPlanOver = FK_PLAN_OVER;
+
+hipError_t planned_capacity(int kc, int32_t *waves) {
+  int dev = 0, cus = 0, nb = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  int wpb = 0;
+  switch (kc) {
+    case 8:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_planned<8>,
+                                                       64 * waves_per_block<8>(), 0);
+      wpb = waves_per_block<8>();
+      break;
+    case 16:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_planned<16>,
+                                                       64 * waves_per_block<16>(), 0);
+      wpb = waves_per_block<16>();
+      break;
+    case 32:
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_planned<32>,
+                                                       64 * waves_per_block<32>(), 0);
+      wpb = waves_per_block<32>();
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  if (e != hipSuccess) return e;
+  // the plan may exceed one resident round by kPlanOver percent: the
+  // lightest super-tiles, planned last, start when the first waves retire
+  const int64_t resident = (int64_t)(nb < 1 ? 1 : nb) * cus * wpb;
+  *waves = (int32_t)((resident * (100 + kPlanOver) / 100) / wpb * wpb);
+  return hipSuccess;
+}
+#endif
 
 hipError_t FK_LAYOUT(launch_candidate_count)(const QueryArgs &a,
                                              unsigned long long *total,

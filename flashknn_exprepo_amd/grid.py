@@ -195,11 +195,11 @@ class Grid:
     def query(self, k, *, mode=F.FK_MODE_CELLS, radius_cells=1, cut_r2=math.inf,
               idx_dtype=torch.int64, pad_mode=F.FK_PAD_SELF, dist_mode=F.FK_DIST_SQ,
               pad_dist=math.inf, attached=False, out_idx=None, out_dist=None,
-              passes=0, prune=False, one_lane=False):
+              passes=0, prune=False, one_lane=False, uniform=False):
         """passes: 0 = full query; else a mask (1 reset, 2 first pass,
-        4 expansion) used to time one kernel alone.  prune / one_lane:
-        FK_QUERY_PRUNE / FK_QUERY_ONE_LANE (speed hints; results are
-        identical)."""
+        4 expansion) used to time one kernel alone.  prune / one_lane /
+        uniform: FK_QUERY_PRUNE / FK_QUERY_ONE_LANE / FK_QUERY_UNIFORM
+        (speed hints; results are identical)."""
         nq = self.num_queries if attached else self.num_points
         if attached and nq < 0:
             raise RuntimeError("no queries attached")
@@ -212,7 +212,8 @@ class Grid:
                           pad_mode, dist_mode, pad_dist,
                           F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
                           passes, (F.FK_QUERY_PRUNE if prune else 0)
-                          | (F.FK_QUERY_ONE_LANE if one_lane else 0))
+                          | (F.FK_QUERY_ONE_LANE if one_lane else 0)
+                          | (F.FK_QUERY_UNIFORM if uniform else 0))
         F.check(self._L.fk_query(self._h, ctypes.byref(p), F.ptr(out_idx),
                                  F.ptr(out_dist), F.stream_ptr()), "fk_query")
         return out_idx, out_dist

@@ -147,6 +147,14 @@ typedef struct fk_query_params {
                                fill the GPU (under 2,048 waves) widens
                                its groups to 4, 8 or 16 lanes per query;
                                results never depend on the group size     */
+#define FK_QUERY_UNIFORM 4 /* first pass in uniform tiles.  Without it, a
+                              self-query launch of 65,536 queries or more
+                              that fits one resident round runs a plan made
+                              once per build: super-tiles of 64 queries
+                              whose 27-cell candidate counts are high get up
+                              to 8 lanes per query, light ones down to 1, so
+                              that no wave's scan runs far longer than the
+                              rest (results never change)                 */
 
 typedef struct fk_grid fk_grid;
 

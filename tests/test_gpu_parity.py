@@ -407,6 +407,37 @@ def test_group_sizes_agree(oracle, n):
                   dg[torch.from_numpy(sample).to(dg.device)], ref_i, ref_d)
 
 
+@pytest.mark.parametrize("cloud,k", [("room", 8), ("room", 16), ("room", 32),
+                                     ("lidar", 16), ("room250k", 16)])
+def test_planned_first_pass_matches_uniform(oracle, cloud, k):
+    """Large self-query launches run the balanced first pass (super-tiles of
+    64 queries on 1, 2, 4 or 8 lanes per query by their candidate counts):
+    every row equals the uniform launch's (FK_QUERY_UNIFORM), in exact and
+    27-cell mode, and brute force on a sample."""
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd import synthetic
+    xyz = {"room": lambda: synthetic.room(seed=48),
+           "lidar": lambda: synthetic.lidar(seed=49),
+           "room250k": lambda: synthetic.room(seed=48, target=250000)}[cloud]()
+    n = len(xyz)
+    e = fk.ExactKNN()
+    e.build(_t(xyz), _t(np.array([n], np.int32)), k)
+    g = e.grid
+    for mode in (F.FK_MODE_EXACT, F.FK_MODE_CELLS):
+        kw = dict(mode=mode, prune=g.prune_hint, one_lane=False)
+        iu, du = g.query(k, uniform=True, **kw)
+        ip, dp = g.query(k, **kw)
+        ip2, dp2 = g.query(k, **kw)  # the plan is kept between calls
+        g.status()
+        assert torch.equal(iu, ip) and torch.equal(du, dp), mode
+        assert torch.equal(ip, ip2) and torch.equal(dp, dp2), mode
+    sample = np.random.default_rng(k).choice(n, 1000, replace=False)
+    ri, rd = oracle.brute_knn(xyz, [n], xyz[sample], [1000], k)
+    ie, de = g.query(k, mode=F.FK_MODE_EXACT, prune=g.prune_hint)
+    s_t = torch.from_numpy(sample).to(ie.device)
+    _same(ie[s_t], de[s_t], ri, rd)
+
+
 def test_cells_prune_matches_oracle(oracle):
     """In the 27-cell mode on an xyz-built grid, pruning returns the same
     rows as the unpruned scan and the oracle's candidate-set kNN."""
