@@ -130,6 +130,57 @@ constexpr uint32_t kInfBits = 0x7f800000u;
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
 
+// The value lane (lane ^ J) holds, J in {1, 2, 4, 8, 16, 32}, without a
+// round trip through LDS (__shfl_xor compiles to ds_bpermute): DPP inside a
+// row of 16 lanes (quad_perm for 1 and 2, row_half_mirror then a reversed
+// quad for 4, row_ror:8 for 8) and the gfx950 row swaps v_permlane16_swap /
+// v_permlane32_swap for 16 and 32.  Every lane of the wave must be active.
+template <int J>
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v) {
+  if constexpr (J == 1) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0xB1, 0xF, 0xF, false);  // quad_perm 1,0,3,2
+  } else if constexpr (J == 2) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x4E, 0xF, 0xF, false);  // quad_perm 2,3,0,1
+  } else if constexpr (J == 4) {
+    const uint32_t h = __builtin_amdgcn_update_dpp(0u, v, 0x141, 0xF, 0xF, false);
+    return __builtin_amdgcn_update_dpp(0u, h, 0x1B, 0xF, 0xF, false);  // quad_perm 3,2,1,0
+  } else if constexpr (J == 8) {
+    return __builtin_amdgcn_update_dpp(0u, v, 0x128, 0xF, 0xF, false);  // row_ror:8
+  } else if constexpr (J == 16) {
+    const auto r = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+    return (threadIdx.x & 16) ? r[0] : r[1];
+  } else {
+    static_assert(J == 32, "lane_xor: J in {1, 2, 4, 8, 16, 32}");
+    const auto r = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+    return (threadIdx.x & 32) ? r[0] : r[1];
+  }
+}
+
+template <int J>
+__device__ __forceinline__ uint64_t lane_xor(uint64_t v) {
+  return ((uint64_t)lane_xor32<J>((uint32_t)(v >> 32)) << 32) | lane_xor32<J>((uint32_t)v);
+}
+
+// lane_xor with the distance known after unrolling.
+__device__ __forceinline__ uint64_t lane_xor_n(uint64_t v, int j) {
+  switch (j) {
+    case 1: return lane_xor<1>(v);
+    case 2: return lane_xor<2>(v);
+    case 4: return lane_xor<4>(v);
+    case 8: return lane_xor<8>(v);
+    case 16: return lane_xor<16>(v);
+    default: return lane_xor<32>(v);
+  }
+}
+
+// The value lane 63 - lane holds: row_mirror inside each row of 16, then
+// the rows in reverse order (row r <- row 3 - r: xor 16, then xor 32).
+__device__ __forceinline__ uint64_t lane_reverse(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_update_dpp(0u, (uint32_t)v, 0x140, 0xF, 0xF, false);
+  const uint32_t hi = __builtin_amdgcn_update_dpp(0u, (uint32_t)(v >> 32), 0x140, 0xF, 0xF, false);
+  return lane_xor<32>(lane_xor<16>(((uint64_t)hi << 32) | lo));
+}
+
 __device__ __forceinline__ float sqdist(float px, float py, float pz, float qx,
                                         float qy, float qz) {
   const float dx = __fsub_rn(px, qx);
@@ -296,7 +347,7 @@ struct TopK {
   __device__ __forceinline__ void share_threshold() {
 #pragma unroll
     for (int o = 1; o < G; o <<= 1) {
-      const uint64_t u = __shfl_xor(thr, o, kWave);
+      const uint64_t u = lane_xor_n(thr, o);
       thr = u < thr ? u : thr;
     }
   }
@@ -963,7 +1014,7 @@ __device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
   for (int k2 = 2; k2 <= kWave; k2 <<= 1) {
 #pragma unroll
     for (int j = k2 >> 1; j > 0; j >>= 1) {
-      const uint64_t o = __shfl_xor(v, j, kWave);
+      const uint64_t o = lane_xor_n(v, j);
       const bool up = (lane & k2) == 0;
       const bool lower = (lane & j) == 0;
       v = (lower == up) ? kmin(v, o) : kmax(v, o);
@@ -974,10 +1025,10 @@ __device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
 
 // L and S ascending across lanes -> the 64 smallest of both, ascending.
 __device__ __forceinline__ uint64_t wave_merge(uint64_t L, uint64_t S, int lane) {
-  uint64_t v = kmin(L, __shfl(S, kWave - 1 - lane, kWave));
+  uint64_t v = kmin(L, lane_reverse(S));
 #pragma unroll
   for (int j = kWave >> 1; j > 0; j >>= 1) {
-    const uint64_t o = __shfl_xor(v, j, kWave);
+    const uint64_t o = lane_xor_n(v, j);
     v = (lane & j) == 0 ? kmin(v, o) : kmax(v, o);
   }
   return v;
@@ -987,7 +1038,7 @@ __device__ __forceinline__ uint64_t wave_merge(uint64_t L, uint64_t S, int lane)
 __device__ __forceinline__ uint64_t wave_bitonic_merge(uint64_t v, int lane) {
 #pragma unroll
   for (int j = kWave >> 1; j > 0; j >>= 1) {
-    const uint64_t o = __shfl_xor(v, j, kWave);
+    const uint64_t o = lane_xor_n(v, j);
     v = (lane & j) == 0 ? kmin(v, o) : kmax(v, o);
   }
   return v;
@@ -1002,13 +1053,20 @@ struct CoopList {
   int k;
   uint64_t cutkey, thr;
 
+  // k is wave-uniform: the k-th key is read from its lane with v_readlane
   __device__ __forceinline__ uint64_t kth() const {
-    if constexpr (P == 1) {
-      return __shfl(L[0], k - 1, kWave);
-    } else {
-      // k is wave-uniform: one of the two reads runs
-      return k <= kWave ? __shfl(L[0], k - 1, kWave) : __shfl(L[1], k - 1 - kWave, kWave);
+    const int r = __builtin_amdgcn_readfirstlane(k - 1);
+    uint64_t v = L[0];
+    int src = r;
+    if constexpr (P > 1) {
+      if (r >= kWave) {
+        v = L[1];
+        src = r - kWave;
+      }
     }
+    const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)v, src);
+    const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)(v >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
   }
   __device__ __forceinline__ void refresh(int lane) {
     (void)lane;
@@ -1026,7 +1084,7 @@ struct CoopList {
         // with +inf to 128): C[i] = min(L[i], S'[127 - i]) is bitonic with
         // C[0..63] = L[0..63]; the first merge stage (distance 64) is the
         // in-lane min/max, then each half merges across the wave.
-        const uint64_t c1 = kmin(L[1], __shfl(S, kWave - 1 - lane, kWave));
+        const uint64_t c1 = kmin(L[1], lane_reverse(S));
         const uint64_t lo = kmin(L[0], c1), hi = kmax(L[0], c1);
         L[0] = wave_bitonic_merge(lo, lane);
         L[1] = wave_bitonic_merge(hi, lane);
@@ -1047,16 +1105,43 @@ struct CoopList {
   }
 };
 
-// Columns of the shell Ro < Chebyshev distance <= Rn, split over the lanes
-// (Ro = -1: the whole block of half-width Rn).
+// Columns of the shell Ro < Chebyshev distance <= Rn (Ro = -1: the whole
+// block of half-width Rn).  Each lane looks up kShellBatch of the shell's
+// columns at a time (their cell-table reads go out together); the records
+// of the batch's non-empty column segments (an inner column has two) are
+// then spread evenly over the wave: the segments go to the wave's LDS table
+// as (record prefix, first record), lane l takes records [l C, (l + 1) C)
+// of their concatenation, C = ceil(total / 64), found by one binary search
+// and then walked in order.  (With one lane per column the wave waited for
+// its longest column: a room query's 15-column shell took 8 dependent
+// rounds of four loads, LiDAR shells through tall objects up to 100.)
 #ifndef FK_SHELL_BATCH
 #define FK_SHELL_BATCH 8
 #endif
 constexpr int kShellBatch = FK_SHELL_BATCH;  // column segments looked up together per lane
+constexpr int kShellTab = 2 * kShellBatch * kWave;  // segment table entries per wave
+
+// Exclusive prefix sum over the wave (every lane active); *total = the sum.
+__device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total) {
+  int x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const int u = __shfl_up(x, o, kWave);
+    if (lane >= o) x += u;
+  }
+  *total = __builtin_amdgcn_readlane(x, kWave - 1);
+  return x - v;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 template <int P>
 __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, const CloudInfo &ci,
-                           int4 qc, float4 qr, int Ro, int Rn, int lane, uint64_t *stab) {
+                           int4 qc, float4 qr, int Ro, int Rn, int lane, uint64_t *wtab) {
   const int x0 = max(-Rn, -qc.x), x1 = min(Rn, ci.nx - 1 - qc.x);
   const int y0 = max(-Rn, -qc.y), y1 = min(Rn, ci.ny - 1 - qc.y);
   const long long wy = y1 >= y0 ? (long long)(y1 - y0 + 1) : 0;
@@ -1073,15 +1158,7 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
   const int wyi = wy > 0 ? (int)(wy < INT_MAX ? wy : INT_MAX) : 1;
   const int step_q = kWave / wyi, step_r = kWave % wyi;
   int cq = lane / wyi, cr = lane % wyi;
-  int p = 0, e = 0;
-  // The lane's columns are looked up kShellBatch at a time: their
-  // cell-table reads go out together and the non-empty ranges (an inner
-  // column has two z segments) are queued in the lane's LDS table.  (One at
-  // a time, a sparse query's large shell -- hundreds of mostly empty columns
-  // per lane -- was a chain of dependent table reads.)
-  uint64_t *tab = stab;
-  int qn = 0, qi = 0;  // queued ranges, next to take
-  auto fill = [&]() {
+  while (wave_any(s < ncol)) {
     int b[2 * kShellBatch], en[2 * kShellBatch];
 #pragma unroll
     for (int j = 0; j < kShellBatch; ++j) {
@@ -1104,41 +1181,63 @@ __device__ __forceinline__ void coop_shell(CoopList<P> &t, const QueryArgs &a, c
       }
     }
     s += (long long)kShellBatch * kWave;
-    qn = 0;
-    qi = 0;
+    int qn = 0, tot = 0;
 #pragma unroll
     for (int j = 0; j < 2 * kShellBatch; ++j)
-      if (b[j] < en[j]) put_range(tab + (qn++) * kWave, b[j], en[j]);
-  };
-  auto advance = [&]() {
-    while (p >= e) {
-      if (qi >= qn) {
-        if (s >= ncol) return;
-        fill();
-        continue;
+      if (b[j] < en[j]) {
+        ++qn;
+        tot += en[j] - b[j];
       }
-      const int2 r = get_range(tab + (qi++) * kWave);
-      p = r.x;
-      e = r.y;
-    }
-  };
-  advance();
-  // four records of the lane's current column in flight per step
-  while (wave_any(p < e)) {
-    float4 r[4];
+    int nr, T;
+    int q = wave_excl_scan(qn, lane, &nr);
+    int pre = wave_excl_scan(tot, lane, &T);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) r[u] = a.srec[p + u < e ? p + u : 0];
+    for (int j = 0; j < 2 * kShellBatch; ++j)
+      if (b[j] < en[j]) {
+        put_range(wtab + q, pre, b[j]);  // (first record's place, first record)
+        pre += en[j] - b[j];
+        ++q;
+      }
+    wave_lds_sync();
+    if (T > 0) {  // wave-uniform
+      const int C = (T + kWave - 1) / kWave;
+      int r = lane * C;
+      const int rend = min(r + C, T);
+      // the segment holding record r: the last i with prefix <= r
+      int lo = 0, hi = nr - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (get_range(wtab + mid).x <= r) lo = mid; else hi = mid - 1;
+      }
+      int i = lo;
+      int2 cur = get_range(wtab + i);
+      int nxt = i + 1 < nr ? get_range(wtab + i + 1).x : T;
+      // four records of the lane's stretch in flight per step
+      while (wave_any(r < rend)) {
+        float4 rec[4];
+        bool has[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const bool has = p + u < e;
-      const uint64_t key =
-          has ? make_key(sqdist(r[u].x, r[u].y, r[u].z, qr.x, qr.y, qr.z),
-                         __float_as_uint(r[u].w))
-              : kMaxKey;
-      t.offer(key, has, lane);
+        for (int u = 0; u < 4; ++u) {
+          has[u] = r + u < rend;
+          while (has[u] && r + u >= nxt) {
+            ++i;
+            cur = get_range(wtab + i);
+            nxt = i + 1 < nr ? get_range(wtab + i + 1).x : T;
+          }
+          rec[u] = a.srec[has[u] ? (uint32_t)(cur.y + (r + u - cur.x)) : 0u];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const uint64_t key =
+              has[u] ? make_key(sqdist(rec[u].x, rec[u].y, rec[u].z, qr.x, qr.y, qr.z),
+                                __float_as_uint(rec[u].w))
+                     : kMaxKey;
+          t.offer(key, has[u], lane);
+        }
+        r += 4;
+      }
     }
-    p = p + 4 < e ? p + 4 : e;
-    if (p >= e) advance();
+    wave_lds_sync();  // the next batch rewrites the table
   }
   if (t.cnt > 0) t.flush(lane);
 }
@@ -1179,11 +1278,11 @@ __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
   }
 }
 
-__global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
+__global__ void __launch_bounds__(64 * kCoopWaves, 4) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
-  __shared__ uint64_t ctab[kCoopWaves * 2 * kShellBatch * kWave];
+  __shared__ uint64_t ctab[kCoopWaves * kShellTab];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  uint64_t *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
+  uint64_t *wave_tab = ctab + w * kShellTab;
   const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq);
   int nexp = 0;
   // static striding over the pending list (a shared work counter measured
@@ -1207,8 +1306,8 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
     if (a.R < Rcover) {
       const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(t, a.cut_r2), Rcover,
                                  coop_found(t, lane), k);
-      coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane, lane_tab);
-      coop_expand<1>(t, a, ci, qc, qr, Rn, lane, lane_tab);
+      coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane, wave_tab);
+      coop_expand<1>(t, a, ci, qc, qr, Rn, lane, wave_tab);
     }
     ++nexp;
     if (lane < k)
@@ -1222,9 +1321,9 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_expand_coop(QueryArgs a) {
 // grows it until the query certifies.
 __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
-  __shared__ uint64_t ctab[kCoopWaves * 2 * kShellBatch * kWave];
+  __shared__ uint64_t ctab[kCoopWaves * kShellTab];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
-  uint64_t *lane_tab = ctab + w * 2 * kShellBatch * kWave + lane;
+  uint64_t *wave_tab = ctab + w * kShellTab;
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
   const int nq = a.scalars[kStatus] != 0 ? 0 : a.nq;
   const int nwaves = gridDim.x * kCoopWaves;
@@ -1243,9 +1342,9 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
     t.L[0] = lane < k ? padkey : kMaxKey;
     t.L[1] = lane + kWave < k ? padkey : kMaxKey;
     t.refresh(lane);
-    coop_shell<2>(t, a, ci, qc, qr, -1, a.R, lane, lane_tab);
+    coop_shell<2>(t, a, ci, qc, qr, -1, a.R, lane, wave_tab);
     if (a.exact) {
-      const int Ro = coop_expand<2>(t, a, ci, qc, qr, a.R, lane, lane_tab);
+      const int Ro = coop_expand<2>(t, a, ci, qc, qr, a.R, lane, wave_tab);
       if (Ro > a.R) ++nexp; else ++ncert;
     }
     const int64_t o = out_base(a, ci, qc.w, __float_as_int(qr.w), k);
