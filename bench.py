@@ -97,7 +97,9 @@ def main():
     build = lambda: g.build_xyz(xyz, off, inv)  # noqa: E731
     build()
     build_ms, _ = events_ms(build, 10)
-    out_i = torch.empty((n, K), dtype=torch.int64, device=dev)
+    # int32 indices (the pointops layout; 4 bytes per neighbour index as in
+    # the algorithmic byte count) and fp32 squared distances
+    out_i = torch.empty((n, K), dtype=torch.int32, device=dev)
     out_d = torch.empty((n, K), dtype=torch.float32, device=dev)
 
     def step():
@@ -153,7 +155,7 @@ def main():
             traffic = json.load(f).get("bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                "traffic": traffic, "kernel": "k_query<16, 2> (first pass)",
+                "traffic": traffic, "kernel": "k_query_planned<16> (first pass, balanced plan)",
                 "kernel_ms": round(p1_ms, 5), "algorithmic_bytes": bytes_q}
 
     if args.copy_mb > 0:
@@ -195,6 +197,25 @@ def main():
         v, ms = qps(gc, K, F.FK_MODE_CELLS)
         extra["room_27cell_alpha4_Mqps"], extra["room_27cell_alpha4_ms"] = round(v, 2), round(ms, 4)
         extra["room_27cell_mean_candidates"] = round(gc.candidate_count(1) / n, 2)
+
+        # orientation at the reference's published size (~250k points): the
+        # 27-cell alpha-4 query (FlashKNN.query semantics) at K=16 and K=32,
+        # and the exact query at K=16
+        log("room 250k")
+        r250 = synthetic.room(seed=48, target=250000)
+        x250 = T(r250)
+        g250 = fk.Grid()
+        g250.build_cells(torch.floor(x250 / 0.04).long(), None, x250, num_down=2)
+        extra["room250k_n"] = len(r250)
+        for kk in (16, 32):
+            v, ms = qps(g250, kk, F.FK_MODE_CELLS)
+            extra[f"room250k_27cell_alpha4_k{kk}_Mqps"] = round(v, 2)
+            extra[f"room250k_27cell_alpha4_k{kk}_ms"] = round(ms, 4)
+        e250 = fk.ExactKNN()
+        e250.build(x250, T(np.array([len(r250)], np.int32)), K)
+        v, ms = qps(e250.grid, K, F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
+        extra["room250k_exact_k16_Mqps"], extra["room250k_exact_k16_ms"] = round(v, 2), round(ms, 4)
+        del g250, e250
 
         # LiDAR sweep
         log("lidar")
@@ -312,17 +333,26 @@ def main():
                 d = torch.cdist(qs[b0:b0 + 10000], s)
                 torch.topk(d, K, dim=1, largest=False)
 
-        cpu_run()
+        for _ in range(3):  # warm-ups
+            cpu_run()
         reps = []
-        for _ in range(3):
+        for _ in range(10):
             t0 = time.perf_counter()
             cpu_run()
             reps.append(time.perf_counter() - t0)
-        tc = sorted(reps)[1]
+        tc = sorted(reps)[len(reps) // 2]
+        model = "unknown"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+        except (OSError, StopIteration):
+            pass
         cpu = {"value": round(qn / tc / 1e6, 5), "unit": "M queries/s",
                "cores": torch.get_num_threads(), "kind": "port",
+               "cpu_model": model, "os_cpu_count": cores,
                "sample": f"{qn} random queries of the {n}-point room against all "
-                         f"{n} points, torch.cdist+topk in blocks of 10000, median of 3"}
+                         f"{n} points, torch.cdist+topk in blocks of 10000, "
+                         f"3 warm-ups + median of 10"}
 
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "M queries/s",
@@ -331,7 +361,8 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"synthetic indoor room, {n} fp32 points (4 cm voxel "
                                "subsampled), queries = all points, K=16, exact kNN "
-                               "(bit-exact idx), single cloud",
+                               "(bit-exact idx; int32 idx + fp32 squared distance "
+                               "written), single cloud",
                    "points": n, "k": K, "parallelism": "single GPU"},
         "roofline": roofline, "cpu_baseline": cpu, "extra": extra,
     }
