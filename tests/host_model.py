@@ -1,0 +1,719 @@
+"""Host model of libflashknn's device-table indexing (TEST INFRASTRUCTURE,
+CPU only).
+
+Replays, on the CPU, what the library does with its device arrays for a
+sequence of Python-level calls -- FlashKNN.query, selected_query,
+back_query, voxel_first and pointops.knn_query (whose exact-kNN engine
+keeps one grid handle, with its reserve, across calls) -- and checks every
+index a kernel computes against the allocation it addresses.  A failed
+check raises ``Violation`` naming the kernel and the index.
+
+What is modelled (file of libflashknn, fk_*.hip):
+  * fk_api.hip   grow / realloc_exact capacities (25% + 64 slack), the
+                 lockstep cell tables (ensure_cells / ensure_qcells),
+                 fk_grid_reserve, the no-sync holds of finish_build and
+                 fk_grid_attach_queries, fk_grid_status, ensure_pending;
+  * fk_build.hip k_init_bbox and the 16 box copies (box_copy, box_read),
+                 k_cells_from_grid / _xyz (domain checks), k_layout (bases,
+                 the 2^30 per-box and capacity clamps, kTotalCells),
+                 k_count + block_rank (LDS hash slots, distinct values per
+                 block), launch_scan (tiles, partials, LDS tile indices,
+                 the out[N] total), k_scatter, k_qcells (clamped buckets,
+                 the extra bucket of clouds without cells), k_qscatter,
+                 the column layout's sort keys;
+  * fk_query.hip column_range reads (dense and column), the record ranges
+                 they return, cover_radius / shell extents, the pending
+                 list (slot guard), the output rows;
+  * fk_voxel.hip the workspace carve and the hash table's slot count.
+The wrappers are mirrored from functions/FlashKnnWrapper.py, knn.py
+(ExactKNN, build_exact_grid) and hierarchy.py (voxel_first).
+
+It models sizes and indices, not results: cell sides follow the library's
+formulas in float64/float32 closely enough to reproduce the build sequence
+(reserve, no-sync overflow, synchronising retry), and every invariant is
+checked for whatever cells the model computes.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+CELL_LIMIT = 1 << 30          # |cell| bound (kCellLimit) and per-box entry bound
+HARD_CELL_CAP = 1 << 30       # kHardCellCap: entries of a synchronising build
+INT_MAX = 2**31 - 1
+SCAN_BLOCK, SCAN_ITEMS = 256, 16
+SCAN_TILE = SCAN_BLOCK * SCAN_ITEMS
+RANK_BLOCK, RANK_SLOTS = 1024, 2048
+OK, ERR_ARG, ERR_K, ERR_TOO_LARGE, ERR_DOMAIN, ERR_HIP, ERR_CAPACITY, ERR_STATE = range(8)
+NO_SYNC, COLUMNS = 1, 2
+MAX_LANE_K = 64
+MAX_K = 128
+
+
+class Violation(AssertionError):
+    """An index outside the allocation it addresses (or a broken invariant
+    a kernel relies on for its indexing)."""
+
+
+class LibError(Exception):
+    def __init__(self, status):
+        super().__init__(status)
+        self.status = status
+
+
+def check(cond, what):
+    if not cond:
+        raise Violation(what)
+
+
+def grow_size(need):
+    return need + need // 4 + 64
+
+
+def scan_parts_needed(n):
+    return (n + 2 + SCAN_TILE - 1) // SCAN_TILE + 1
+
+
+def box_copies(B):
+    return 16 if B <= 1024 else 1
+
+
+def box_stride(B):
+    return (6 * B + 31) // 32 * 32
+
+
+def box_words(B):
+    small = 16 * box_stride(1024)
+    if B <= 1024:
+        return box_copies(B) * box_stride(B)
+    return max(box_stride(B), small)
+
+
+def scan_pad(i):
+    return i + (i >> 4)
+
+
+def bits(v):
+    b = 0
+    while v >> b:
+        b += 1
+    return b
+
+
+class Handle:
+    """fk_grid: element capacities of its device arrays, and build state."""
+
+    def __init__(self):
+        self.cap = {}
+        self.max_clouds = 0
+        self.reserved = False
+        self.built = False
+        self.from_xyz = False
+        self.n_points = 0
+        self.n_clouds = 0
+        self.n_cells = -1
+        self.n_queries = -1
+        self.build_cap = 0
+        self.columns = False
+        self.dev_status = OK
+        self.total = 0
+        self.boxes = []          # per cloud: (base, nx, ny, nz, minx, miny, minz)
+        self.cells = None        # absolute cells [n, 3] and cloud [n] of the build
+        self.cloud = None
+        self.q = None            # attached (local cells [nq, 3], cloud [nq])
+
+    # ---------------------------------------------------------- capacities
+    def c(self, name):
+        return self.cap.get(name, 0)
+
+    def grow(self, name, need):
+        if need <= self.c(name) and self.c(name) > 0:
+            return
+        self.cap[name] = grow_size(need)
+
+    def ensure_clouds(self, B):
+        self.grow("bbox", box_words(B))
+        self.grow("clouds", B)
+        self.grow("prefix", B + 1)
+        self.grow("acc", B)
+        self.max_clouds = max(self.max_clouds, B)
+
+    def ensure_points(self, n):
+        m = max(n, 1)
+        for name in ("pt_cell", "pt_lin", "pt_rank", "srec", "scell"):
+            self.grow(name, m)
+
+    def ensure_cells(self, cells):
+        if self.c("counts") and cells + 1 <= self.c("counts"):
+            return
+        c = cells + cells // 4 + 64
+        self.cap["counts"] = c + 1
+        self.cap["cell_start"] = c + 2
+        self.grow("scan_part", scan_parts_needed(c + 2))
+
+    def ensure_qcells(self, cells):
+        if self.c("qcounts") and cells + 2 <= self.c("qcounts"):
+            return
+        c = cells + cells // 4 + 64
+        self.cap["qcounts"] = c + 2
+        self.cap["qstart"] = c + 3
+        self.grow("scan_part", scan_parts_needed(c + 3))
+
+    def ensure_queries(self, nq):
+        m = max(nq, 1)
+        for name in ("qrec", "qcell", "q_cell_abs", "qpad", "q_lin", "q_rank"):
+            self.grow(name, m)
+
+    def ensure_pending(self, nq, k):
+        m = max(nq, 1)
+        self.grow("pend_q", m)
+        self.grow("pend_keys", m * k)
+
+    def reserve(self, max_points, max_cells, max_clouds, max_queries):
+        if (max_points < 0 or max_cells < 0 or max_clouds < 1 or max_queries < 0
+                or max_points > INT_MAX or max_cells > HARD_CELL_CAP):
+            raise LibError(ERR_ARG)
+        self.ensure_clouds(max_clouds)
+        self.ensure_points(max_points)
+        self.ensure_cells(max_cells)
+        self.ensure_qcells(max_cells)
+        nq = max(max_queries, max_points)
+        self.ensure_queries(nq)
+        self.ensure_pending(nq, MAX_LANE_K)
+        self.reserved = True
+
+    # ------------------------------------------------------------- kernels
+    def _k_boxes(self, n, B):
+        """k_init_bbox, the cell kernels' box_copy and k_layout's box_read."""
+        copies, stride = box_copies(B), box_stride(B)
+        check(copies & (copies - 1) == 0, "box copies not a power of two")
+        last_init = ((copies * B - 1) // B) * stride + 6 * ((copies * B - 1) % B) + 5
+        check(last_init < self.c("bbox"), f"k_init_bbox word {last_init} of {self.c('bbox')}")
+        nblocks = max(1, -(-n // 256))
+        last_copy = min(nblocks - 1, copies - 1) if n else copies - 1
+        check(last_copy * stride + 6 * (B - 1) + 5 < self.c("bbox"), "box_copy past the box buffer")
+        check((copies - 1) * stride + 6 * (B - 1) + 5 < self.c("bbox"), "box_read past the box buffer")
+        check(n <= self.c("pt_cell"), f"pt_cell[{n - 1}] of {self.c('pt_cell')}")
+
+    def _scan(self, in_name, out_name, N, hint, extra):
+        """launch_scan: tiles of hint + extra + 1 elements, partials, and the
+        exclusive scan out[0, N] of in[0, N)."""
+        nb = max(1, -(-(hint + extra + 1) // SCAN_TILE))
+        check(nb <= self.c("scan_part"), f"scan partials {nb} of {self.c('scan_part')}")
+        check(N <= nb * SCAN_TILE - 1, f"scan total out[{N}] beyond the launch's tiles")
+        check(N <= self.c(in_name), f"scan reads {in_name}[{N - 1}] of {self.c(in_name)}")
+        check(N + 1 <= self.c(out_name), f"scan writes {out_name}[{N}] of {self.c(out_name)}")
+        check(scan_pad(SCAN_TILE - 1) < scan_pad(SCAN_TILE), "scan LDS tile index")
+
+    def _layout(self, cell, cloud, B, columns, bb_cap, status):
+        st = status
+        boxes = []
+        carry = 0
+        max_depth = 0
+        for c in range(B):
+            m = cloud == c
+            base = min(carry, INT_MAX)
+            if not m.any():
+                boxes.append((base, 0, 0, 0, 0, 0, 0))
+                continue
+            lo = cell[m].min(0)
+            hi = cell[m].max(0)
+            ex, ey, ez = (int(v) for v in hi - lo + 1)
+            entries = ex * ey if columns else ex * ey * ez
+            if entries > CELL_LIMIT:
+                st = st or ERR_TOO_LARGE
+                boxes.append((base, 0, 0, 0, 0, 0, 0))
+                continue
+            if columns:
+                max_depth = max(max_depth, ez)
+            boxes.append((base, ex, ey, ez, int(lo[0]), int(lo[1]), int(lo[2])))
+            carry += entries
+        if carry > bb_cap:
+            st = st or ERR_TOO_LARGE
+            kt = min(bb_cap, INT_MAX)
+        else:
+            kt = carry
+        return st, boxes, kt, max_depth
+
+    def _lin(self, cell, cloud, boxes, columns, clamp=False):
+        out = np.full(len(cell), -1, np.int64)
+        for c, (base, nx, ny, nz, mx, my, mz) in enumerate(boxes):
+            m = cloud == c
+            if not m.any() or nx == 0:
+                continue
+            l = cell[m] - np.array([mx, my, mz])
+            if clamp:
+                l = np.clip(l, 0, np.array([nx, ny, nz]) - 1)
+            else:
+                check((l >= 0).all() and (l < np.array([nx, ny, nz])).all(),
+                      "k_count: point outside its cloud's box")
+            out[m] = base + l[:, 0] * ny + l[:, 1] if columns else \
+                base + (l[:, 0] * ny + l[:, 1]) * nz + l[:, 2]
+        return out
+
+    def start(self, e):
+        """cell_start[e] of the last build: points in entries below e."""
+        return int(np.searchsorted(self._lin_sorted, e, side="left"))
+
+    def _count_scatter(self, n, cell, cloud, kt, hint, st):
+        check(kt <= self.c("counts"), f"k_fill_dev counts[{kt - 1}] of {self.c('counts')}")
+        if st == OK and n:
+            lin = self._lin(cell, cloud, self.boxes, self.columns)
+            check(lin.min() >= 0 and lin.max() < kt, "k_count: table entry outside [0, kTotalCells)")
+            # block_rank: one LDS hash table of RANK_SLOTS per block of RANK_BLOCK points
+            for b0 in range(0, n, RANK_BLOCK):
+                d = len(np.unique(lin[b0:b0 + RANK_BLOCK]))
+                check(d < RANK_SLOTS, "block_rank: more distinct values than LDS slots")
+            slot_hi = (((INT_MAX * 2654435761) & 0xffffffff) >> 21)
+            check(slot_hi < RANK_SLOTS, "block_rank slot index")
+            self._lin_sorted = np.sort(lin)
+        self._scan("counts", "cell_start", kt, hint, 0)
+        if st == OK and n:
+            # k_scatter: pos = cell_start[lin] + rank
+            check(self.start(kt) == n and n <= self.c("srec") and n <= self.c("scell"),
+                  "k_scatter positions past the sorted records")
+
+    def build(self, cell, cloud, B, flags, from_xyz):
+        """fk_grid_build_cells / _xyz + finish_build.  cell: absolute int64
+        cells [n, 3]; cloud: int64 [n], -1 where the cell kernels raise
+        FK_ERR_DOMAIN.  Returns the status a synchronising call reports."""
+        n = len(cell)
+        nosync = bool(flags & NO_SYNC)
+        if nosync and flags & COLUMNS:
+            raise LibError(ERR_ARG)
+        if nosync:
+            if n > self.c("pt_cell") or B > self.max_clouds or not self.c("counts"):
+                raise LibError(ERR_CAPACITY)
+        else:
+            self.ensure_clouds(B)
+            self.ensure_points(n)
+            if not self.c("counts"):
+                self.ensure_cells(1024)
+        self._k_boxes(n, B)
+        status = OK if (cloud >= 0).all() else ERR_DOMAIN
+        ok = cloud >= 0
+        cell_ok, cloud_ok = cell[ok], cloud[ok]
+        self.from_xyz = from_xyz
+        bb_cap = self.c("counts") - 1
+        if nosync:
+            if self.c("qcounts") and self.c("qcounts") - 1 < bb_cap:
+                bb_cap = self.c("qcounts") - 1
+        else:
+            bb_cap = HARD_CELL_CAP
+        columns = bool(flags & COLUMNS)
+        st, boxes, kt, depth = self._layout(cell_ok, cloud_ok, B, columns, bb_cap, status)
+        check(B <= self.c("clouds"), "k_layout: clouds[] past its array")
+        if nosync:
+            hint = bb_cap
+            self.n_cells = -1
+        else:
+            if st == ERR_TOO_LARGE and not columns:
+                st, boxes, kt, depth = self._layout(cell_ok, cloud_ok, B, True, bb_cap, OK)
+                columns = True
+            if st != OK:
+                self.dev_status, self.boxes, self.total, self.columns = st, boxes, kt, columns
+                return st
+            hint = kt
+            self.ensure_cells(hint)
+            self.n_cells = hint
+        check(kt <= hint, "kTotalCells above the build's hint")
+        self.dev_status, self.boxes, self.total, self.columns = st, boxes, kt, columns
+        self.cells, self.cloud = cell, cloud
+        self._count_scatter(n, cell, cloud, kt, hint, st)
+        if columns:
+            depth_bits = bits(depth - 1 if depth > 0 else 0)
+            key_bits = max(1, depth_bits + bits(hint))
+            check(key_bits <= 64, "column sort keys wider than 64 bits")
+            for name in ("ckeys0", "ckeys1", "cperm0", "cperm1"):
+                self.grow(name, max(n, 1))
+            if st == OK and n:
+                lin = self._lin(cell, cloud, boxes, True)
+                lz = np.zeros(n, np.int64)
+                for c, (base, nx, ny, nz, mx, my, mz) in enumerate(boxes):
+                    m = cloud == c
+                    lz[m] = cell[m, 2] - mz
+                check(((lin << depth_bits) | lz).max() < (1 << key_bits), "column key overflow")
+        self.build_cap = hint
+        self.built = True
+        self.n_points, self.n_clouds, self.n_queries = n, B, -1
+        self.q = None
+        return OK
+
+    def status(self):
+        """fk_grid_status: reports the device status; n_cells after a no-sync build."""
+        if self.dev_status != OK:
+            return self.dev_status
+        if self.built:
+            self.n_cells = self.total
+        return OK
+
+    def attach(self, qcell, qcloud, flags):
+        """fk_grid_attach_queries: qcell absolute [nq, 3], qcloud [nq] (-1:
+        domain error)."""
+        nq = len(qcell)
+        if not self.built:
+            raise LibError(ERR_STATE)
+        nosync = bool(flags & NO_SYNC)
+        hint = self.n_cells
+        if nosync or hint < 0:
+            if nq > self.c("qrec") or not self.c("qcounts") or self.build_cap > self.c("qcounts") - 1:
+                raise LibError(ERR_CAPACITY)
+            hint = self.build_cap
+        else:
+            self.ensure_queries(nq)
+            self.ensure_qcells(hint)
+        kt = self.total
+        check(kt + 1 <= self.c("qcounts"), f"k_fill_dev qcounts[{kt}] of {self.c('qcounts')}")
+        check(nq <= self.c("q_lin"), "k_qcells past the query arrays")
+        st = self.dev_status
+        if st == OK and (qcloud < 0).any():
+            st = ERR_DOMAIN
+        if st == OK and nq:
+            lin = np.full(nq, kt, np.int64)
+            local = np.zeros((nq, 3), np.int64)
+            for c, (base, nx, ny, nz, mx, my, mz) in enumerate(self.boxes):
+                m = qcloud == c
+                if not m.any():
+                    continue
+                l = np.clip(qcell[m] - np.array([mx, my, mz]), -CELL_LIMIT, CELL_LIMIT)
+                local[m] = l
+                if nx:
+                    lin[m] = self._lin(qcell[m], np.zeros(m.sum(), np.int64),
+                                       [self.boxes[c]], self.columns, clamp=True)
+            check(lin.min() >= 0 and lin.max() <= kt, "k_qcells bucket outside [0, kTotalCells]")
+            check(lin.max() < self.c("qcounts"), "k_qcells bucket past qcounts")
+            self._scan("qcounts", "qstart", kt + 1, hint, 1)
+            # k_qscatter: qstart[lin] + rank < (queries in buckets <= lin) <= nq
+            check(len(lin) == nq and nq <= self.c("qrec"), "k_qscatter past the query records")
+            self.q = (local, qcloud)
+        else:
+            self._scan("qcounts", "qstart", kt + 1, hint, 1)
+            self.q = None
+        self.n_queries = nq
+        if not nosync and st != OK:
+            self.n_queries = -1
+            self.dev_status = st
+            return st
+        self.dev_status = st
+        return OK
+
+    def _reads(self, local, qcloud, R):
+        """Start-table reads of column_range over the block of half-width R
+        and over the expansion's shells up to cover_radius: the largest index
+        read must stay below the cell_start array and the record range below
+        n.  Returns the largest cover radius (the expansion's bound)."""
+        rmax = 0
+        n = self.n_points
+        for c, (base, nx, ny, nz, mx, my, mz) in enumerate(self.boxes):
+            m = qcloud == c
+            if not m.any() or nx == 0:
+                continue
+            l = local[m]
+            # column_range clamps cx, cy to the box and z to [0, nz); the last
+            # entry read is the end of the box's last segment
+            hi = base + (nx * ny if self.columns else nx * ny * nz)
+            check(hi <= self.total and hi < self.c("cell_start"),
+                  f"column_range reads cell_start[{hi}] of {self.c('cell_start')}")
+            check(self.start(hi) <= n, "column_range: record range past n")
+            cov = np.maximum.reduce([l[:, 0], nx - 1 - l[:, 0], l[:, 1], ny - 1 - l[:, 1],
+                                     l[:, 2], nz - 1 - l[:, 2]])
+            cov = np.maximum(cov, 0)
+            check(cov.max() <= INT_MAX, "cover_radius beyond int")
+            # coop_shell: x0 = max(-Rn, -qc.x), x1 = min(Rn, nx - 1 - qc.x)
+            # in int: qc.x + Rn must not wrap for queries inside the box
+            inside = (l >= 0).all(1) & (l < np.array([nx, ny, nz])).all(1)
+            if inside.any():
+                check((l[inside].max(1) + cov[inside]).max() <= INT_MAX, "shell column index wraps")
+            rmax = max(rmax, int(cov.max()))
+        return rmax
+
+    def query(self, k, exact, attached, R=1):
+        """fk_query: pending lists, first-pass / expansion reads."""
+        if k < 1 or k > MAX_K:
+            raise LibError(ERR_K)
+        if not self.built:
+            raise LibError(ERR_STATE)
+        if attached and self.n_queries < 0:
+            raise LibError(ERR_STATE)
+        nq = self.n_queries if attached else self.n_points
+        if exact and k <= MAX_LANE_K:
+            self.ensure_pending(nq, k)
+            # slot guard: slot < nq, keys at slot * k + j, j < k
+            check(nq <= self.c("pend_q") and nq * k <= self.c("pend_keys"), "pending lists")
+        if self.dev_status != OK:
+            return  # every query kernel exits on the status word
+        if attached:
+            if self.q is None:
+                return
+            local, qcloud = self.q
+        else:
+            local = np.zeros((self.n_points, 3), np.int64)
+            for c, (base, nx, ny, nz, mx, my, mz) in enumerate(self.boxes):
+                m = self.cloud == c
+                local[m] = self.cells[m] - np.array([mx, my, mz])
+            qcloud = self.cloud
+        self._reads(local, qcloud, R)
+
+
+# ------------------------------------------------------------ cell helpers
+
+def _cloud_of(off, n):
+    """cloud_of: first c with offsets[c] > i (B for points past the last end)."""
+    return np.searchsorted(np.asarray(off, np.int64), np.arange(n), side="right")
+
+
+def xyz_cells(xyz, off, inv):
+    """k_cells_from_xyz: floor(x * inv[c]) in fp32; -1 cloud on domain errors."""
+    n, B = len(xyz), len(off)
+    cl = _cloud_of(off, n)
+    ok = cl < B
+    inv_p = np.asarray(inv, np.float32)[np.minimum(cl, B - 1)]
+    u = (xyz.astype(np.float32) * inv_p[:, None]).astype(np.float32)
+    f = np.floor(u)
+    ok &= (np.abs(f) < np.float32(CELL_LIMIT)).all(1)
+    cell = np.where(ok[:, None], f, 0).astype(np.int64)
+    return cell, np.where(ok, cl, -1).astype(np.int64)
+
+
+def auto_cell(xyz, off, k, scale=1.0):
+    """k_auto_cell: side per cloud from the float bounding box (double math)."""
+    B = len(off)
+    side = np.ones(B, np.float64)
+    start = 0
+    for c in range(B):
+        p = xyz[start:off[c]].astype(np.float64)
+        n = len(p)
+        if n:
+            ext = p.max(0) - p.min(0)
+            span = max(ext.max(), 1e-6)
+            vol = float(np.prod(np.maximum(ext, 1e-2 * span)))
+            s = scale * (k * vol / n) ** (1.0 / 3.0)
+            s = max(s, (vol / (64.0 * n)) ** (1.0 / 3.0), 1e-6)
+            side[c] = s
+        start = off[c]
+    s32 = side.astype(np.float32)
+    return s32, (1.0 / s32.astype(np.float64)).astype(np.float32)
+
+
+def cloud_candidates(h):
+    """fk_grid_cloud_candidates: per-cloud sum of 27-cell candidate counts
+    (0 when the device status is set)."""
+    B = h.n_clouds
+    out = np.zeros(B, np.int64)
+    if h.dev_status != OK or not h.n_points:
+        return out
+    cell, cloud = h.cells, h.cloud
+    key = lambda c, b: ((b * (1 << 21) + (c[:, 0] + (1 << 20))) * (1 << 21)
+                        + (c[:, 1] + (1 << 20))) * (1 << 21) + (c[:, 2] + (1 << 20))
+    span = np.abs(cell).max() if len(cell) else 0
+    if span >= (1 << 19):
+        return out  # the model's key packing; such clouds keep their first guess
+    keys = key(cell, cloud)
+    u, cnt = np.unique(keys, return_counts=True)
+    tot = np.zeros(len(cell), np.int64)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                kk = key(cell + np.array([dx, dy, dz]), cloud)
+                pos = np.searchsorted(u, kk)
+                pos = np.minimum(pos, len(u) - 1)
+                tot += np.where(u[pos] == kk, cnt[pos], 0)
+    np.add.at(out, cloud, tot)
+    return out
+
+
+# ------------------------------------------------------------- wrappers
+
+def flashknn_query(h, grid, batch, k, nd):
+    """FlashKNN.query: synchronising build from cells (cell = grid >> nd)."""
+    n = len(grid)
+    B = int(batch.max()) + 1 if batch is not None and n else 1
+    cell = grid >> nd
+    cloud = np.zeros(n, np.int64) if batch is None else batch.astype(np.int64)
+    bad = (np.abs(cell) >= CELL_LIMIT).any(1) | (cloud < 0) | (cloud >= B)
+    cloud = np.where(bad, -1, cloud)
+    st = h.build(cell, cloud, B, 0, False)
+    if st != OK:
+        raise LibError(st)
+    h.query(k, False, False)
+
+
+def selected_query(h, grid, batch, qi, k, nd):
+    """FlashKNN.selected_query: build as query, attach grid[qi] (sync)."""
+    flashknn_query(h, grid, batch, k, nd)
+    qc = grid[qi] >> nd
+    qb = np.zeros(len(qi), np.int64) if batch is None else batch[qi].astype(np.int64)
+    qb = np.where((qb < 0) | (qb >= h.n_clouds), -1, qb)
+    st = h.attach(qc, qb, 0)
+    if st != OK:
+        raise LibError(st)
+    h.query(k, False, True)
+
+
+def back_query(h, xyz, sel, qgs, k):
+    """FlashKNN.back_query (one cloud): cells trunc(fl32(x / fl32(size)))."""
+    q = (xyz.astype(np.float32) / np.float32(qgs)).astype(np.float32)
+    grid = np.trunc(q).astype(np.int64)
+    st = h.build(grid[sel], np.zeros(len(sel), np.int64), 1, 0, False)
+    if st != OK:
+        raise LibError(st)
+    st = h.attach(grid, np.zeros(len(xyz), np.int64), 0)
+    if st != OK:
+        raise LibError(st)
+    h.query(k, False, True)
+
+
+def voxel_first(n, B=1):
+    """fk_voxel_first: the workspace carve and the hash table's slots."""
+    check(B <= 32767, "voxel key holds 15 cloud bits")
+    S = 64
+    while S < 2 * n:
+        S <<= 1
+    check(S >= 2 * n and S & (S - 1) == 0, "voxel table below 2n slots")
+    al = lambda v: (v + 255) & ~255
+    need = 256 + al(8 * S) + al(4 * S) + 2 * al(4 * n) + al(4 * (n + 1)) + al(4 * scan_parts_needed(n))
+    nb = max(1, -(-(n + 1) // SCAN_TILE))
+    check(nb <= scan_parts_needed(n), "voxel scan partials past the workspace")
+    return need
+
+
+class Engine:
+    """knn.ExactKNN: one handle, its reserve, no-sync builds with the density
+    feedback, and the synchronising retry after FK_ERR_GRID_TOO_LARGE."""
+
+    def __init__(self):
+        self.h = Handle()
+        self._cap = (0, 0, 0)
+        self.retries = 0
+
+    def _reserve(self, n, B, nq):
+        need = (n, B, nq)
+        if all(a <= c for a, c in zip(need, self._cap)):
+            return
+        n2, B2, q2 = (max(int(a * 1.25) + 16, c) for a, c in zip(need, self._cap))
+        cells = min(128 * n2 + 16 * B2 + 65536, 1 << 28)
+        self.h.reserve(n2, cells, B2, max(n2, q2))
+        self._cap = (n2, B2, q2)
+
+    def _build(self, xyz, off, k, no_sync):
+        from flashknn_exprepo_amd.autotune import lookup_target_ratio
+        from flashknn_exprepo_amd.knn import (ADAPT_MIN_POINTS, ADAPT_ROUNDS,
+                                              CANDIDATE_EXPONENT, ONE_SHOT_BAND,
+                                              density_class)
+        h = self.h
+        n, B = len(xyz), len(off)
+        if no_sync:
+            self._reserve(n, B, n)
+        flags = NO_SYNC if no_sync else 0
+        side, inv = auto_cell(xyz, off, k)
+        self._apply(h.build(*xyz_cells(xyz, off, inv), B, flags, True))
+        if n < ADAPT_MIN_POINTS:
+            return
+        sizes = np.maximum(np.diff(np.concatenate([[0], off])), 1)
+        mean = cloud_candidates(h) / sizes
+        ratios = [lookup_target_ratio(int(nc), k, density_class(mc, k))
+                  for nc, mc in zip(sizes, mean)]
+        if all(1.0 / ONE_SHOT_BAND <= mc / (r * k) <= ONE_SHOT_BAND
+               for mc, r in zip(mean, ratios)):
+            return
+        for _ in range(ADAPT_ROUNDS):
+            acc = cloud_candidates(h)
+            for c in range(B):  # k_adapt
+                base, nx, ny, nz = h.boxes[c][:4] if c < len(h.boxes) else (0, 0, 0, 0)
+                npts = int(sizes[c])
+                if acc[c] == 0:
+                    continue
+                f = (ratios[c] * k / (acc[c] / npts)) ** (1.0 / CANDIDATE_EXPONENT)
+                f = min(max(f, 0.25), 4.0)
+                f = max(f, (float(nx) * ny * nz / (64.0 * npts)) ** (1.0 / 3.0))
+                side[c] = np.float32(float(side[c]) * f)
+                inv[c] = np.float32(1.0 / float(side[c]))
+            self._apply(h.build(*xyz_cells(xyz, off, inv), B, flags, True))
+
+    def _apply(self, st):
+        if st != OK:
+            raise LibError(st)
+
+    def __call__(self, xyz, off, k):
+        try:
+            self._run(xyz, off, k, True)
+        except LibError as e:
+            if e.status != ERR_TOO_LARGE:
+                raise
+            self.retries += 1
+            self._run(xyz, off, k, False)
+
+    def _run(self, xyz, off, k, no_sync):
+        self._build(xyz, off, k, no_sync)
+        self.h.query(k, True, False)
+        st = self.h.status()
+        if st != OK:
+            raise LibError(st)
+
+
+# ---------------------------------------------------- randomised sweep
+
+_BASE = {}
+
+
+def _base():
+    if not _BASE:
+        from flashknn_exprepo_amd import synthetic
+        _BASE["room"] = synthetic.room(seed=48, target=30000)
+        _BASE["lidar"] = synthetic.lidar(seed=49)
+    return _BASE
+
+
+def _cloud(rng, m):
+    from flashknn_exprepo_amd import synthetic
+    kind = int(rng.integers(0, 4))
+    if kind == 0:
+        x = rng.random((m, 3)) * rng.uniform(0.5, 10)
+    elif kind == 1:
+        x = synthetic.subsample_to(_base()["room"], m, rng)
+    elif kind == 2:
+        x = synthetic.subsample_to(_base()["lidar"], m, rng)
+    else:
+        x = np.round(rng.random((m, 3)) * 4) / 4
+    return x.astype(np.float32)
+
+
+def sweep_case(seed):
+    """The case of the randomised GPU sweep (tests/test_gpu_random.py) for
+    one seed, drawn in the same order from default_rng(1000 + seed)."""
+    rng = np.random.default_rng(1000 + seed)
+    B = int(rng.integers(1, 4))
+    parts = [_cloud(rng, int(rng.integers(200, 8000))) for _ in range(B)]
+    xyz = np.concatenate(parts)
+    batch = np.concatenate([np.full(len(p), i) for i, p in enumerate(parts)]).astype(np.int64)
+    vox = float(rng.choice([0.02, 0.05, 0.1, 0.25]))
+    grid = np.floor(xyz / np.float32(vox)).astype(np.int64)
+    if rng.random() < 0.3:
+        grid = grid - grid.min(0) + rng.integers(0, 100, 3)
+    nd = int(rng.integers(0, 3))
+    k = int(rng.choice([1, 2, 4, 8, 16, 24, 32, 48, 64]))
+    _cut = math.inf if rng.random() < 0.7 else float(rng.uniform(0.05, 1.0))
+    qi = np.sort(rng.choice(len(xyz), int(rng.integers(1, len(xyz))), replace=False))
+    kb = int(rng.choice([1, 3, 8]))
+    qgs = vox * float(rng.choice([2.0, 4.0]))
+    kk = int(rng.choice([1, 3, 16, 32, 100, 128]))
+    return dict(parts=parts, xyz=xyz, batch=batch if B > 1 else None, vox=vox, grid=grid,
+                nd=nd, k=k, qi=qi, kb=kb, qgs=qgs, kk=kk)
+
+
+def run_case(engine, case, sel=None):
+    """All five calls of one sweep case; ``sel`` is voxel_first's selection
+    on the first cloud (the oracle's, passed in by the caller)."""
+    h = Handle()  # a new FlashKNN object per case: one grid for query + selected
+    flashknn_query(h, case["grid"], case["batch"], case["k"], case["nd"])
+    selected_query(h, case["grid"], case["batch"], case["qi"], case["k"], case["nd"])
+    x1 = case["parts"][0]
+    voxel_first(len(x1))
+    if sel is not None:
+        try:
+            back_query(Handle(), x1, sel, case["qgs"], case["kb"])
+        except LibError:
+            pass
+    off = np.cumsum([len(p) for p in case["parts"]]).astype(np.int64)
+    engine(case["xyz"], off, case["kk"])
