@@ -746,7 +746,9 @@ int fk_query_stats(fk_grid *g, int64_t *first_pass_certified, int64_t *expanded,
   FK_CHECK(check_stream_device(g));
   FK_CHECK(read_scalars(g, S(stream)));
   const int32_t *c = g->h_pinned + kCounters + kCntStride * g->stat_set;
-  const int64_t pending = c[kCntPending] < g->stat_nq ? c[kCntPending] : g->stat_nq;
+  const int64_t inl = c[kCntInline] < kInlineBudget ? c[kCntInline] : kInlineBudget;
+  const int64_t listed = (int64_t)c[kCntPending] + inl;
+  const int64_t pending = listed < g->stat_nq ? listed : g->stat_nq;
   if (first_pass_certified) *first_pass_certified = g->stat_nq - pending;
   if (expanded) *expanded = c[kCntExpanded];
   return ok();

@@ -30,10 +30,18 @@ enum Scalar : int {
 // pending one.)
 enum Counter : int {
   kCntPending = 0,    // exact mode: queries left uncertified by pass 1
-  kCntExpanded = 2,   // exact mode: resolved by shell expansion
-  kCntStride = 4      // slots per set (slots 1 and 3 spare)
+  kCntInline = 1,     // exact mode: uncertified ones the first pass expanded itself
+  kCntExpanded = 2,   // exact mode: resolved by shell expansion (both kinds)
+  kCntStride = 4      // slots per set (slot 3 spare)
 };
 constexpr int kBatchedSet = 2;
+// Uncertified queries a plain first pass may expand itself per launch
+// (fk_query.hip, run_tile); every claim counts in kCntInline, those past the
+// budget are listed for the expansion launch instead.
+#ifndef FK_INLINE_BUDGET
+#define FK_INLINE_BUDGET 8
+#endif
+constexpr int kInlineBudget = FK_INLINE_BUDGET;
 
 // Cell box of one cloud.  Dense layout: cell (lx,ly,lz) (local coordinates,
 // 0 <= l < n) has table entry base + (lx*ny + ly)*nz + lz, so the three cells

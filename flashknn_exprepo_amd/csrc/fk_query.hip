@@ -85,6 +85,11 @@ namespace {
 #define FK_WPE_64 1
 #endif
 //   FK_G_<KC>        lanes per query in the first pass (1, 2 or 4)
+//   FK_INLINE_PLAIN  the unplanned first pass (k_query) expands a tile's
+//                    lone uncertified query itself, as the planned one does
+#ifndef FK_INLINE_PLAIN
+#define FK_INLINE_PLAIN 1
+#endif
 #ifndef FK_G_8
 #define FK_G_8 2
 #endif
@@ -129,6 +134,14 @@ constexpr int min_waves_per_eu() {
 constexpr uint32_t kInfBits = 0x7f800000u;
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// LDS writes of the wave visible to its own later LDS reads.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 
 // The value lane (lane ^ J) holds, J in {1, 2, 4, 8, 16, 32}, without a
 // round trip through LDS (__shfl_xor compiles to ds_bpermute): DPP inside a
@@ -886,12 +899,29 @@ __device__ __forceinline__ int wave_prefix_slot(bool pred, int32_t *counter) {
   return base + __popcll(below);
 }
 
+// An uncertified query the first pass finishes itself instead of listing it
+// for the expansion launch: one per tile at most, only when it is the tile's
+// only one and already holds k candidates (its next shell is sized from its
+// k-th distance), and at most kInlineBudget per launch.  That one query's
+// shells are a chain of dependent reads (~7 us alone in the expansion
+// launch) which a first-pass wave hides behind the other waves; many such
+// queries, or ones still searching for k candidates, would lengthen the
+// first pass instead.
+
+template <int KC>
+__device__ __forceinline__ uint64_t *inline_seed(uint64_t *wave_buf, uint64_t *wave_tab);
+template <int KC>
+__device__ __forceinline__ void inline_expand(const QueryArgs &a, int q, int lane,
+                                              uint64_t *wave_buf, uint64_t *wave_tab);
+
 // One wave: queries [q0, q0 + 64 / G) of the sorted query list, G lanes per
 // query, all of one K class.  Used by the plain kernel and by the persistent
-// batched kernel.
-template <int KC, int G>
-__device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
-                                         uint64_t *lane_buf, uint64_t *lane_tab) {
+// batched kernel.  With INL (the plain kernel) it returns the sorted index of
+// a query left for the wave to expand (its list in the wave's seed area), or
+// -1.
+template <int KC, int G, bool INL = false>
+__device__ __forceinline__ int run_tile(const QueryArgs &a, int q0, int qend,
+                                        uint64_t *lane_buf, uint64_t *lane_tab) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sub = lane & (G - 1);
   const int qi = q0 + lane / G;
@@ -918,6 +948,28 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
     // certified count is the query count minus the pending one, so certified
     // queries touch no shared counter)
     const bool lead = sub == 0;
+    if constexpr (INL && kInlineBudget > 0) {
+      const uint64_t mu = __ballot(active && !done && lead);
+      const bool found = key_dbits(t.kth()) != kInfBits;
+      if (__popcll(mu) == 1 && __ballot(active && !done && lead && found) == mu) {
+        int claim = 0;
+        if (lane == 0) claim = atomicAdd(a.qcnt + kCntInline, 1);
+        claim = __builtin_amdgcn_readfirstlane(claim);
+        if (claim < kInlineBudget) {  // wave-uniform
+          const int lead_lane = __builtin_ctzll(mu);
+          uint64_t *seed = inline_seed<KC>(lane_buf - lane, lane_tab - lane);
+          wave_lds_sync();  // the tile's last merge has read its buffers
+          if (lane == lead_lane) {
+#pragma unroll
+            for (int j = 0; j < KC; ++j)
+              if (j >= t.base()) seed[j - t.base()] = t.top[j];
+          }
+          if (active && done)
+            write_row<KC, G>(a, t, out_base(a, ci, qc.w, __float_as_int(qr.w), k), sub);
+          return __builtin_amdgcn_readlane(qs, lead_lane);
+        }
+      }
+    }
     int slot = wave_prefix_slot(active && !done && lead, a.qcnt + kCntPending);
     if constexpr (G > 1) slot = __shfl(slot, lane & ~(G - 1), kWave);
     if (active && !done && slot >= a.nq) {
@@ -936,6 +988,7 @@ __device__ __forceinline__ void run_tile(const QueryArgs &a, int q0, int qend,
     const int row = __float_as_int(qr.w);
     write_row<KC, G>(a, t, out_base(a, ci, qc.w, row, k), sub);
   }
+  return -1;
 }
 
 template <int KC>
@@ -958,8 +1011,10 @@ k_query(QueryArgs a) {
   // wave-uniform exits: past the end, or the build/attach raised an error
   // (its tables may then be incomplete)
   if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
-  run_tile<KC, G>(a, q0, a.nq, sbuf + w * BS * kWave + lane,
-                  stab + w * stream_entries<KC>() * kWave + lane);
+  uint64_t *wb = sbuf + w * BS * kWave, *wt = stab + w * stream_entries<KC>() * kWave;
+  const int qx = run_tile<KC, G, (FK_INLINE_PLAIN != 0)>(a, q0, a.nq, wb + lane, wt + lane);
+  if constexpr (FK_INLINE_PLAIN != 0)
+    if (qx >= 0) inline_expand<KC>(a, qx, lane, wb, wt);
 }
 
 // The balanced first pass: wave w runs plan[w] = (first query, G) with G
@@ -982,15 +1037,16 @@ k_query_planned(QueryArgs a) {
   const int q0 = __builtin_amdgcn_readfirstlane(d.x);
   const int g = __builtin_amdgcn_readfirstlane(d.y);
   if (q0 >= a.nq) return;
-  uint64_t *lb = sbuf + w * BS * kWave + lane;
-  uint64_t *lt = stab + w * stream_entries<KC>() * kWave + lane;
+  uint64_t *wb = sbuf + w * BS * kWave, *wt = stab + w * stream_entries<KC>() * kWave;
   const int qend = min(a.nq, q0 + kWave / g);
+  int qx;
   switch (g) {
-    case 1: run_tile<KC, 1>(a, q0, qend, lb, lt); break;
-    case 2: run_tile<KC, 2>(a, q0, qend, lb, lt); break;
-    case 4: run_tile<KC, 4>(a, q0, qend, lb, lt); break;
-    default: run_tile<KC, 8>(a, q0, qend, lb, lt); break;
+    case 1: qx = run_tile<KC, 1, true>(a, q0, qend, wb + lane, wt + lane); break;
+    case 2: qx = run_tile<KC, 2, true>(a, q0, qend, wb + lane, wt + lane); break;
+    case 4: qx = run_tile<KC, 4, true>(a, q0, qend, wb + lane, wt + lane); break;
+    default: qx = run_tile<KC, 8, true>(a, q0, qend, wb + lane, wt + lane); break;
   }
+  if (qx >= 0) inline_expand<KC>(a, qx, lane, wb, wt);
 }
 
 // ------------------------------------------------- cooperative expansion
@@ -1131,12 +1187,6 @@ __device__ __forceinline__ int wave_excl_scan(int v, int lane, int *total) {
   }
   *total = __builtin_amdgcn_readlane(x, kWave - 1);
   return x - v;
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int P>
@@ -1314,6 +1364,59 @@ __global__ void __launch_bounds__(64 * kCoopWaves, 4) k_expand_coop(QueryArgs a)
       write_entry(a, out_base(a, ci, qc.w, __float_as_int(qr.w), k) + lane, t.L[0]);
   }
   if (lane == 0 && nexp) atomicAdd(a.qcnt + kCntExpanded, nexp);
+}
+
+// The tile's LDS (key buffers, stream table) once its scan is done: the
+// shell table at the start of the key buffers, then the list buffer and the
+// seed (the inline query's first-pass list), in the key buffers when they
+// are long enough, else in the stream table.
+template <int KC>
+struct InlineLds {
+  static constexpr int kBuf = buffer_slots<KC>() * kWave;
+  static constexpr int kTab = stream_entries<KC>() * kWave;
+  static constexpr bool kInBuf = kBuf >= kShellTab + kCoopCap + kWave;
+  static_assert(kBuf >= kShellTab, "tile buffers below the shell table");
+  static_assert(kInBuf || kTab >= kCoopCap + kWave, "tile LDS below the list buffer");
+  static __device__ __forceinline__ uint64_t *coop(uint64_t *wave_buf, uint64_t *wave_tab) {
+    return kInBuf ? wave_buf + kShellTab : wave_tab;
+  }
+};
+
+template <int KC>
+__device__ __forceinline__ uint64_t *inline_seed(uint64_t *wave_buf, uint64_t *wave_tab) {
+  return InlineLds<KC>::coop(wave_buf, wave_tab) + kCoopCap;
+}
+
+// The query run_tile left to its wave (sorted index q, list in the seed
+// area), finished as k_expand_coop finishes a listed one: the shells grow
+// until it certifies, then its row is written.
+template <int KC>
+__device__ __forceinline__ void inline_expand(const QueryArgs &a, int q, int lane,
+                                              uint64_t *wave_buf, uint64_t *wave_tab) {
+  uint64_t *coop = InlineLds<KC>::coop(wave_buf, wave_tab);
+  uint64_t *seed = coop + kCoopCap;
+  wave_lds_sync();  // the seed is written
+  const float4 qr = a.qrec[q];
+  const int4 qc = a.qcell[q];
+  const CloudInfo ci = a.clouds[qc.w];
+  const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
+  CoopList<1> c;
+  c.buf = coop;
+  c.cnt = 0;
+  c.k = k;
+  c.cutkey = make_key(a.cut_r2, 0u);
+  c.L[0] = lane < k ? seed[lane] : kMaxKey;
+  c.refresh(lane);
+  const int Rcover = cover_radius(ci, qc);
+  if (a.R < Rcover) {
+    const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(c, a.cut_r2), Rcover,
+                               coop_found(c, lane), k);
+    coop_shell<1>(c, a, ci, qc, qr, a.R, Rn, lane, wave_buf);
+    coop_expand<1>(c, a, ci, qc, qr, Rn, lane, wave_buf);
+  }
+  if (lane < k)
+    write_entry(a, out_base(a, ci, qc.w, __float_as_int(qr.w), k) + lane, c.L[0]);
+  if (lane == 0) atomicAdd(a.qcnt + kCntExpanded, 1);
 }
 
 // k in (64, 128]: one wave per query from the start, the best list held as
