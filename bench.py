@@ -104,8 +104,7 @@ def main():
 
     def step():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
-                out_dist=out_d, prune=g.prune_hint,
-                one_lane=g.one_lane_hint)
+                out_dist=out_d, **g.hints())
 
     log(f"headline: n={n} K={K} warmup={args.warmup} steps={args.steps}")
     for _ in range(args.warmup):
@@ -129,8 +128,7 @@ def main():
     # dominant kernel alone (first pass), HIP events on the current stream
     def pass1():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
-                out_dist=out_d, passes=2, prune=g.prune_hint,
-                one_lane=g.one_lane_hint)
+                out_dist=out_d, passes=2, **g.hints())
 
     def reset():
         g.query(K, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1, out_idx=out_i,
@@ -171,15 +169,14 @@ def main():
              "certified_first_pass": stats["certified_first_pass"],
              "expanded": stats["expanded"], "bitexact_sample_2000": exact_ok,
              "cell_side_m": round(float(cell[0]), 5), "prune": bool(g.prune_hint),
-             "one_lane": bool(g.one_lane_hint)}
+             "one_lane": bool(g.one_lane_hint), "candidate_batch": g.batch_hint}
 
     if not args.quick:
         def qps(grid, k, mode, reps=20, **kw):
             oi = torch.empty((grid.num_points, k), dtype=torch.int64, device=dev)
             od = torch.empty((grid.num_points, k), dtype=torch.float32, device=dev)
             f = lambda: grid.query(k, mode=mode, out_idx=oi, out_dist=od,  # noqa: E731
-                                   prune=grid.prune_hint,
-                                   one_lane=grid.one_lane_hint, **kw)
+                                   **grid.hints(), **kw)
             f()
             med, _ = events_ms(f, reps)
             return grid.num_points / (med * 1e-3) / 1e6, med
@@ -274,7 +271,7 @@ def main():
         def per_cloud():
             for gr, k, oi, od, _ in engines:
                 gr.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_SELF, out_idx=oi,
-                         out_dist=od, prune=gr.prune_hint, one_lane=gr.one_lane_hint)
+                         out_dist=od, **gr.hints())
 
         pc_med, _ = events_ms(per_cloud, 20)
         same = True

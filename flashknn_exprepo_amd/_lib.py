@@ -42,6 +42,8 @@ FK_BUILD_COLUMNS = 2
 FK_QUERY_PRUNE = 1
 FK_QUERY_ONE_LANE = 2
 FK_QUERY_UNIFORM = 4
+FK_QUERY_BATCH_SMALL = 8
+FK_QUERY_BATCH_LARGE = 16
 FK_VOXEL_NOALLOC_MAX_POINTS = 1048576
 
 

@@ -1,4 +1,5 @@
-"""Autotuner for the grid cell size of exact queries.
+"""Autotuner for exact queries: grid cell size, queries per wavefront
+and candidate batch.
 
 The kernel's cost per query is (candidates scanned) + (merges) + (the share
 of queries the first pass cannot certify, which pay a shell expansion).
@@ -6,10 +7,14 @@ Larger cells certify more queries but scan more candidates.  The cell side
 is set by density feedback to give ``target_ratio * k`` candidates per
 query in the 27-cell block; the best ratio depends on the cloud size, k and
 how the points fill space.  ``tune`` times the exact query for a few ratios
-on the actual workload, with and without two speed hints (pruning of
-neighbour cells, one or two lanes per query = 64 or 32 queries per wave),
-and records the choice per (log2 N, k class, density class) bucket in a
-JSON table; ``lookup_*`` read it (falling back to the defaults).
+on the actual workload, with and without three speed hints (pruning of
+neighbour cells; one or two lanes per query = 64 or 32 queries per wave;
+the candidate batch, i.e. the records a lane loads per round before the
+wave checks its LDS key buffers: small / default / large), and records the
+choice per (log2 N, k class, density class) bucket in a JSON table;
+``lookup_*`` read it (falling back to the defaults).  Results never depend
+on any of these choices (the reference's rule for its tile variants,
+Query/benchmark_thread_grouping.py:308-311, tightened to bit-identical).
 """
 from __future__ import annotations
 
@@ -26,6 +31,8 @@ TABLE_PATH = os.environ.get(
     "FLASHKNN_TUNE_TABLE",
     os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_table.json"))
 RATIOS = (4.5, 6.5, 9.0, 13.0, 19.0, 28.0)
+BATCHES = ("small", "default", "large")
+SIZES = "120000,60000,30000,10000,4000,2000"
 
 
 def _kclass(k: int) -> int:
@@ -82,6 +89,14 @@ def lookup_one_lane(n: int, k: int, dclass: int = 0) -> bool:
     return bool(e.get("one_lane", False)) if e else False
 
 
+def lookup_batch(n: int, k: int, dclass: int = 0) -> str:
+    """Tuned candidate batch ("small" / "default" / "large"); else the
+    default."""
+    e = _entry(n, k, dclass)
+    b = e.get("batch", "default") if e else "default"
+    return b if b in BATCHES else "default"
+
+
 def _time_ms(fn, reps=11):
     fn()
     torch.cuda.synchronize()
@@ -95,16 +110,19 @@ def _time_ms(fn, reps=11):
 
 
 def _label(key):
-    r, pr, one = key
-    return f"{r}{'p' if pr else ''}{'1' if one else ''}"
+    r, pr, one, b = key
+    return f"{r}{'p' if pr else ''}{'1' if one else ''}{'' if b == 'default' else b[0]}"
 
 
-def tune(xyz, offsets, k, ratios=RATIOS, save=True):
+def tune(xyz, offsets, k, ratios=RATIOS, save=True, check=True):
     """Time the exact query of one cloud at each target ratio, with and
-    without FK_QUERY_PRUNE (k <= 16) and with one or two lanes per query
-    (FK_QUERY_ONE_LANE, k <= 32); record the best under the cloud's (size, k,
-    density class) bucket: the largest ratio within 3 % of the fastest.
-    Returns ((ratio, prune, one_lane), {key: ms}, density class)."""
+    without FK_QUERY_PRUNE (k <= 16), with one or two lanes per query
+    (FK_QUERY_ONE_LANE, k <= 32) and with each candidate batch (k <= 32);
+    record the best under the cloud's (size, k, density class) bucket: the
+    largest ratio within 3 % of the fastest (then the default batch, then
+    the fastest).  With ``check`` every variant's indices and distances must
+    equal the first variant's at the same ratio, bit for bit.
+    Returns ((ratio, prune, one_lane, batch), {key: ms}, density class)."""
     from .knn import ExactKNN, density_class
     eng = ExactKNN()
     eng.build(xyz, offsets, k, adapt=False)
@@ -112,23 +130,36 @@ def tune(xyz, offsets, k, ratios=RATIOS, save=True):
     dclass = density_class(float(eng.grid.cloud_candidates()[0]) / max(n, 1), k)
     results = {}
     for r in ratios:
-        eng.build(xyz, offsets, k, target_ratio=r)
+        # every size adapts here (single small clouds keep their first guess
+        # at run time, but their buckets size the clouds of batched builds)
+        eng.build(xyz, offsets, k, target_ratio=r, min_points=0)
+        ref = None
         for pr in ((False, True) if k <= 16 else (False,)):
             for one in ((False, True) if k <= 32 else (False,)):
-                results[(r, pr, one)] = _time_ms(
-                    lambda: eng.grid.query(k, mode=1, pad_mode=1, prune=pr, one_lane=one))
+                for b in (BATCHES if k <= 32 else ("default",)):
+                    kw = dict(prune=pr, one_lane=one, batch=b)
+                    results[(r, pr, one, b)] = _time_ms(
+                        lambda: eng.grid.query(k, mode=1, pad_mode=1, **kw))
+                    if check:
+                        got = eng.grid.query(k, mode=1, pad_mode=1, **kw)
+                        if ref is None:
+                            ref = got
+                        elif not (torch.equal(got[0], ref[0]) and
+                                  torch.equal(got[1].view(torch.int32),
+                                              ref[1].view(torch.int32))):
+                            raise RuntimeError(f"variant {kw} at ratio {r} changed the result")
     # within 3 % of the fastest, take the largest cells: they leave fewer
     # queries to the shell expansion, which costs most where many clouds
     # share one launch (the batched kernel)
     fastest = min(results.values())
     best = max((key for key, v in results.items() if v <= 1.03 * fastest),
-               key=lambda key: (key[0], -results[key]))
+               key=lambda key: (key[0], key[3] == "default", -results[key]))
     if save:
         with _LOCK:
             t = _load()
             t[bucket(n, k, dclass)] = {
                 "target_ratio": best[0], "prune": best[1], "one_lane": best[2],
-                "density_class": dclass,
+                "batch": best[3], "density_class": dclass,
                 "ms": {_label(key): round(v, 4) for key, v in results.items()}}
             tmp = TABLE_PATH + ".tmp"
             with open(tmp, "w") as f:
@@ -138,32 +169,33 @@ def tune(xyz, offsets, k, ratios=RATIOS, save=True):
 
 
 def main(argv=None):
-    """Tune the synthetic room / LiDAR workloads at a few sizes and k and
-    write the table: ``python -m flashknn_exprepo_amd.autotune --out PATH``."""
+    """Tune the synthetic room / LiDAR workloads at the sizes of the batched
+    workload (2k-120k points, each generated at that size) and k in {8, 16,
+    32}, and write the table:
+    ``python -m flashknn_exprepo_amd.autotune --out PATH``."""
     import argparse
-
-    import numpy as np
 
     from . import synthetic
 
     global TABLE_PATH, _TABLE
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=TABLE_PATH)
-    ap.add_argument("--sizes", default="100000,30000,10000")
+    ap.add_argument("--sizes", default=SIZES)
     ap.add_argument("--ks", default="8,16,32")
     args = ap.parse_args(argv)
     TABLE_PATH, _TABLE = args.out, {}
     dev = torch.device("cuda", 0)
-    bases = {"room": synthetic.room(seed=48), "lidar": synthetic.lidar(seed=49)}
-    for name, base in bases.items():
+    makers = {"room": synthetic.room_points, "lidar": synthetic.lidar_points}
+    for name, make in makers.items():
         for n in (int(s) for s in args.sizes.split(",")):
-            pts = synthetic.subsample_to(base, n, np.random.default_rng(n))
+            pts = make(n, 48 if name == "room" else 49)
             x = torch.from_numpy(pts).to(dev)
             o = torch.tensor([len(pts)], dtype=torch.int32, device=dev)
             for k in (int(s) for s in args.ks.split(",")):
                 best, res, dc = tune(x, o, k)
                 print(f"{name} n={len(pts)} k={k} d={dc}: best ratio {best[0]}"
                       f"{' prune' if best[1] else ''}{' one-lane' if best[2] else ''}"
+                      f" batch {best[3]}"
                       f" ({res[best]:.4f} ms; " +
                       ", ".join(f"{_label(key)}:{m:.4f}" for key, m in sorted(res.items()))
                       + ")", flush=True)

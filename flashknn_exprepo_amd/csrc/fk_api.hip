@@ -129,6 +129,11 @@ struct fk_grid {
   // captured batched step
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // batched first pass: the K classes above the first run on these streams
+  // (default priority), forked from and joined back to the caller's stream
+  hipStream_t aux[kBatchedClasses - 1] = {};
+  hipEvent_t ev_fork = nullptr;
+  hipEvent_t ev_join[kBatchedClasses - 1] = {};
   // balanced first pass of self queries: per-wave plan, super-tile costs,
   // and what the plan was computed for
   int2 *plan = nullptr;
@@ -202,7 +207,8 @@ int bind_device(fk_grid *g) {
 int ensure_clouds(fk_grid *g, int64_t B) {
   FK_CHECK(grow(g, g->bbox, g->cap_bbox, box_words(B), "bbox"));
   FK_CHECK(grow(g, g->clouds, g->cap_clouds, B, "clouds"));
-  FK_CHECK(grow(g, g->item_prefix, g->cap_prefix, B + 1, "item prefix"));
+  FK_CHECK(grow(g, g->item_prefix, g->cap_prefix, kBatchedClasses * (B + 1),
+                "item prefix"));
   FK_CHECK(grow(g, g->cloud_acc, g->cap_acc, B, "per-cloud sums"));
   if (B > g->max_clouds) g->max_clouds = B;
   return FK_OK;
@@ -440,6 +446,11 @@ int fk_grid_destroy(fk_grid *g) {
   };
   if (g->gexec) note(hipGraphExecDestroy(g->gexec));
   if (g->graph) note(hipGraphDestroy(g->graph));
+  for (int i = 0; i < kBatchedClasses - 1; ++i) {
+    if (g->aux[i]) note(hipStreamDestroy(g->aux[i]));
+    if (g->ev_join[i]) note(hipEventDestroy(g->ev_join[i]));
+  }
+  if (g->ev_fork) note(hipEventDestroy(g->ev_fork));
   void *bufs[] = {g->scalars, g->bbox, g->clouds, g->item_prefix, g->cloud_acc,
                   g->pt_cell,
                   g->pt_lin, g->pt_rank, g->srec, g->scell, g->counts,
@@ -647,6 +658,8 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
     return fail(FK_ERR_INVALID_ARG, "fk_query: out_dist is NULL");
   if (std::isnan(p->cut_r2))
     return fail(FK_ERR_INVALID_ARG, "fk_query: cut_r2 is NaN");
+  if ((p->flags & FK_QUERY_BATCH_SMALL) && (p->flags & FK_QUERY_BATCH_LARGE))
+    return fail(FK_ERR_INVALID_ARG, "fk_query: FK_QUERY_BATCH_SMALL and _LARGE together");
   memset(&a, 0, sizeof(a));
   a.srec = g->srec;
   a.scell = g->scell;
@@ -671,6 +684,7 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.passes = p->passes ? (p->passes & 7) : 7;
   a.prune = (p->flags & FK_QUERY_PRUNE) ? 1 : 0;
   a.one_lane = (p->flags & FK_QUERY_ONE_LANE) ? 1 : 0;
+  a.batch_sel = (p->flags & FK_QUERY_BATCH_SMALL) ? 1 : (p->flags & FK_QUERY_BATCH_LARGE) ? 2 : 0;
   a.scalars = g->scalars;
   a.qcnt = g->scalars + kCounters + kCntStride * g->qset;
   a.qcnt_next = nullptr;
@@ -690,7 +704,7 @@ static int attach_plan(fk_grid *g, const fk_query_params *p, QueryArgs &a,
     return FK_OK;
   const int kc = kclass_of(a.k);
   int32_t cap = 0;
-  FK_HIP(planned_capacity(kc, &cap));
+  FK_HIP(planned_capacity(kc, a.batch_sel, &cap));
   const int64_t ntiles = ((int64_t)a.nq + kPlanTile - 1) / kPlanTile;
   if (ntiles > cap || ntiles > kPlanMaxTilesHost) return FK_OK;
   FK_CHECK(grow(g, g->plan, g->cap_plan, cap, "first-pass plan"));
@@ -891,16 +905,57 @@ static int batched_args(fk_grid *g, const fk_query_params *p,
   return FK_OK;
 }
 
+// Work items, then one persistent first-pass kernel per K class up to
+// max_k's (the classes above the first forked onto the handle's auxiliary
+// streams, so that one class's last tiles overlap the others' instead of
+// leaving the GPU idle), joined back on s, then the shell expansion of the
+// pending queries of all classes.  Graph-capturable (the fork and join are
+// events).
+#ifndef FK_BATCHED_FORK
+#define FK_BATCHED_FORK 1
+#endif
+// The auxiliary streams and events (created once, on the handle's device,
+// outside any capture).
+static int ensure_fork(fk_grid *g) {
+  if (!FK_BATCHED_FORK || g->ev_fork) return FK_OK;
+  for (int i = 0; i < kBatchedClasses - 1; ++i) {
+    if (!g->aux[i]) FK_HIP(hipStreamCreateWithFlags(&g->aux[i], hipStreamNonBlocking));
+    if (!g->ev_join[i])
+      FK_HIP(hipEventCreateWithFlags(&g->ev_join[i], hipEventDisableTiming));
+  }
+  FK_HIP(hipEventCreateWithFlags(&g->ev_fork, hipEventDisableTiming));
+  return FK_OK;
+}
+
+static int enqueue_batched(fk_grid *g, const QueryArgs &a, int32_t max_k, hipStream_t s) {
+  FK_HIP(launch_items(g->clouds, g->cell_start, a.k_per_cloud, g->n_clouds, max_k,
+                      g->item_prefix, g->scalars, a.qcnt, s));
+  const int top = max_k <= 8 ? 0 : (max_k <= 16 ? 1 : (max_k <= 32 ? 2 : 3));
+  if (FK_BATCHED_FORK && top > 0) {
+    FK_HIP(hipEventRecord(g->ev_fork, s));
+    for (int cls = top; cls >= 1; --cls) {
+      hipStream_t as = g->aux[cls - 1];
+      FK_HIP(hipStreamWaitEvent(as, g->ev_fork, 0));
+      FK_HIP(launch_batched_first(a, cls, as));
+      FK_HIP(hipEventRecord(g->ev_join[cls - 1], as));
+    }
+    FK_HIP(launch_batched_first(a, 0, s));
+    for (int cls = top; cls >= 1; --cls) FK_HIP(hipStreamWaitEvent(s, g->ev_join[cls - 1], 0));
+  } else {
+    for (int cls = top; cls >= 0; --cls) FK_HIP(launch_batched_first(a, cls, s));
+  }
+  FK_HIP(a.columns ? launch_expand_columns(a, s) : launch_expand(a, s));
+  return FK_OK;
+}
+
 int fk_batched_query(fk_grid *g, const fk_query_params *p,
                      const int32_t *k_per_cloud, const int64_t *out_offsets,
                      void *out_idx, float *out_dist, void *stream) {
   FK_CHECK(check_stream_device(g));
   QueryArgs a;
   FK_CHECK(batched_args(g, p, k_per_cloud, out_offsets, out_idx, out_dist, a));
-  hipStream_t s = S(stream);
-  FK_HIP(launch_items(g->clouds, g->cell_start, g->n_clouds, p->k, g->item_prefix,
-                      g->scalars, a.qcnt, s));
-  FK_HIP(launch_batched(a, p->k, s));
+  FK_CHECK(ensure_fork(g));
+  FK_CHECK(enqueue_batched(g, a, p->k, S(stream)));
   g->stat_set = kBatchedSet;
   g->batched_nq = a.exact ? a.nq : 0;
   g->stat_nq = g->batched_nq;
@@ -923,17 +978,16 @@ int fk_batched_capture(fk_grid *g, const fk_query_params *p,
     FK_HIP(hipGraphDestroy(g->graph));
     g->graph = nullptr;
   }
+  FK_CHECK(ensure_fork(g));
   hipStream_t s = S(stream);
   FK_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  hipError_t e1 = launch_items(g->clouds, g->cell_start, g->n_clouds, p->k,
-                               g->item_prefix, g->scalars, a.qcnt, s);
-  hipError_t e2 = e1 == hipSuccess ? launch_batched(a, p->k, s) : e1;
+  const int rc = enqueue_batched(g, a, p->k, s);
   hipGraph_t graph = nullptr;
   hipError_t e3 = hipStreamEndCapture(s, &graph);
-  if (e2 != hipSuccess || e3 != hipSuccess) {
+  if (rc != FK_OK || e3 != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
-    return fail(FK_ERR_HIP, "graph capture failed: %s",
-                hipGetErrorString(e2 != hipSuccess ? e2 : e3));
+    if (rc != FK_OK) return rc;
+    return fail(FK_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e3));
   }
   g->graph = graph;
   FK_HIP(hipGraphInstantiate(&g->gexec, graph, nullptr, nullptr, 0));

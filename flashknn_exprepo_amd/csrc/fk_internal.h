@@ -12,13 +12,16 @@ constexpr int kWave = 64;
 // k above this runs one wave per query (fk_query) and is refused by the
 // batched kernel
 constexpr int kMaxLaneK = 64;
+// K classes of the batched launch (k up to 8, 16, 32, 64), one persistent
+// kernel and one work queue each
+constexpr int kBatchedClasses = 4;
 
 // Slots of the small device scalar block owned by every grid handle.
 enum Scalar : int {
   kStatus = 0,        // first error raised by a kernel (fk_status), 0 = ok
   kTotalCells = 1,    // cells of all per-cloud boxes
   kMaxDepth = 2,      // column layouts: largest box depth nz of a cloud
-  kQueueHead = 3,     // batched mode: next work item
+  kQueueHead = 3,     // batched mode: next work item of K class c at kQueueHead + c
   kCounters = 16,     // query counter sets: kCounters + kCntStride * set
   kNumScalars = 32
 };
@@ -246,6 +249,8 @@ struct QueryArgs {
   int32_t passes;        // bit 0 reset, bit 1 first pass, bit 2 expansion
   int32_t prune;         // FK_QUERY_PRUNE: prune neighbour cells mid-scan
   int32_t one_lane;      // FK_QUERY_ONE_LANE: one lane per query (64 per wave)
+  int32_t batch_sel;     // candidate batch: 0 default, 1 FK_QUERY_BATCH_SMALL,
+                         // 2 FK_QUERY_BATCH_LARGE
   int32_t columns;       // the grid uses the column layout (see CloudInfo)
   int32_t *scalars;
   int32_t *qcnt;         // this call's counter set (Counter slots)
@@ -276,22 +281,43 @@ hipError_t launch_query(const QueryArgs &a, hipStream_t s);
 // one round after another); the plan gives those more lanes and light ones
 // fewer.  Results do not depend on the plan.
 constexpr int kPlanTile = 64;
-// Resident waves of the planned first pass of K class kc on this device.
-hipError_t planned_capacity(int kc, int32_t *waves);
+// Resident waves of the planned first pass of K class kc and candidate
+// batch batch_sel (QueryArgs::batch_sel) on this device.
+hipError_t planned_capacity(int kc, int batch_sel, int32_t *waves);
 // Writes plan[0, capacity) (first query, lanes); tile_cost: int32[ceil(nq /
 // kPlanTile)] scratch.  Needs nq <= kPlanTile * capacity.
 hipError_t launch_plan(const QueryArgs &a, int32_t capacity, int32_t *tile_cost,
                        int2 *plan, hipStream_t s);
 hipError_t launch_plan_columns(const QueryArgs &a, int32_t capacity,
                                int32_t *tile_cost, int2 *plan, hipStream_t s);
-hipError_t launch_batched(const QueryArgs &a, int32_t max_k, hipStream_t s);
+// First pass of K class KC (8, 16, 32, 64) on the dense / column layout;
+// each is instantiated in its class's translation unit of fk_query.hip.
+template <int KC>
+hipError_t launch_first_pass(const QueryArgs &a, hipStream_t s);
+template <int KC>
+hipError_t launch_first_pass_columns(const QueryArgs &a, hipStream_t s);
+// Persistent batched first pass over the work items of K class KC (queue
+// and prefix slot cls); launch_batched_first dispatches on cls.
+template <int KC>
+hipError_t launch_batched_class(const QueryArgs &a, int cls, hipStream_t s);
+template <int KC>
+hipError_t launch_batched_class_columns(const QueryArgs &a, int cls, hipStream_t s);
+hipError_t launch_batched_first(const QueryArgs &a, int cls, hipStream_t s);
+hipError_t launch_batched_first_columns(const QueryArgs &a, int cls, hipStream_t s);
+// Resident blocks per CU of the planned first pass of K class KC.
+template <int KC>
+hipError_t planned_blocks(int batch_sel, int *nb);
+// Shell expansion of the pending (uncertified) queries of an exact launch,
+// and the first pass of k > kMaxLaneK (one wave per query), per layout.
+hipError_t launch_expand(const QueryArgs &a, hipStream_t s);
+hipError_t launch_expand_columns(const QueryArgs &a, hipStream_t s);
+hipError_t launch_wide(const QueryArgs &a, hipStream_t s);
+hipError_t launch_wide_columns(const QueryArgs &a, hipStream_t s);
 
 int kclass_of(int k);  // 8, 16, 32, 64 or 128
 // The query launches built for the column layout (fk_query.hip compiled with
 // FK_COLUMNS=1); the launches above forward to them when a.columns is set.
 hipError_t launch_query_columns(const QueryArgs &a, hipStream_t s);
-hipError_t launch_batched_columns(const QueryArgs &a, int32_t max_k,
-                                  hipStream_t s);
 hipError_t launch_candidate_count_columns(const QueryArgs &a,
                                           unsigned long long *total,
                                           hipStream_t s);
@@ -320,12 +346,13 @@ hipError_t launch_adapt_cell(const QueryArgs &a, unsigned long long *per_cloud,
                              hipStream_t s);
 hipError_t launch_cloud_candidates(const QueryArgs &a, unsigned long long *acc,
                                    hipStream_t s);
-// Per-cloud work-item prefix of a batched launch whose largest k is max_k
-// (one item = batched_tile(max_k) queries).
-int batched_tile(int32_t max_k);
+// Per-class work-item prefixes of a batched launch ([kBatchedClasses][B +
+// 1]: an item is one tile of a cloud of that K class), queue heads reset;
+// a k_per_cloud entry outside [1, max_k] raises FK_ERR_K_RANGE.
 hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
-                        int32_t B, int32_t max_k, int32_t *prefix,
-                        int32_t *scalars, int32_t *qcnt, hipStream_t s);
+                        const int32_t *k_per_cloud, int32_t B, int32_t max_k,
+                        int32_t *prefix, int32_t *scalars, int32_t *qcnt,
+                        hipStream_t s);
 int64_t scan_parts_needed(int64_t n);
 hipError_t launch_scan_host(const int32_t *in, int32_t *out, int64_t n,
                             int32_t *part, hipStream_t s);

@@ -42,6 +42,7 @@
 
 #include <cfloat>
 #include <climits>
+#include <type_traits>
 
 #ifndef FK_COLUMNS
 #define FK_COLUMNS 0
@@ -51,6 +52,24 @@
 #else
 #define FK_LAYOUT(name) name
 #endif
+
+// Translation units: the file is compiled once per part and layout, so the
+// parts build in parallel: FK_QPART = 8, 16, 32 or 64 holds that K class's
+// first-pass and batched kernels and their launchers, 2 the cooperative
+// expansion, 3 the one-wave-per-query kernel of k > 64, and 1 the rest
+// (counts, the plan, work items, the dispatching launchers); FK_QPART = 0
+// compiles all of it as one unit.
+#ifndef FK_QPART
+#define FK_QPART 0
+#endif
+#define FK_COMMON (FK_QPART == 0 || FK_QPART == 1)
+#define FK_EXPAND_PART (FK_QPART == 0 || FK_QPART == 2)
+#define FK_WIDE_PART (FK_QPART == 0 || FK_QPART == 3)
+#define FK_CLASS(kc) (FK_QPART == 0 || FK_QPART == (kc))
+// Each part (and the column layout) uses only some of the shared helpers.
+#pragma clang diagnostic ignored "-Wunused-function"
+#pragma clang diagnostic ignored "-Wunused-const-variable"
+#pragma clang diagnostic ignored "-Wunneeded-internal-declaration"
 
 namespace fk {
 
@@ -420,29 +439,30 @@ struct TopK {
 //
 // With G lanes per query, lane `sub` of the group takes records
 // b + sub, b + sub + G, ... (the group's loads are adjacent).
-template <int KC, int G>
+template <int KC, int G, int U = kUnroll>
 __device__ __forceinline__ void scan_range(TopK<KC> &t, int b, int e, int sub,
                                            float qx, float qy, float qz,
                                            const float4 *__restrict__ srec) {
-  constexpr int kRoom = buffer_slots<KC>() - kUnroll;
+  constexpr int kRoom = buffer_slots<KC>() - U;
+  static_assert(kRoom >= 0, "buffer too small for a round");
   const int span = e - b - sub;
   const int len = span > 0 ? (span + G - 1) / G : 0;  // this lane's records
   const int first = b + sub;
   int s = 0;
   for (;;) {
     while (wave_any(s < len) && !wave_any(t.cnt > kRoom)) {
-      float4 r[kUnroll];
+      float4 r[U];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
+      for (int u = 0; u < U; ++u) {
         const bool ok = s + u < len;
         r[u] = srec[ok ? (uint32_t)(first + (s + u) * G) : 0u];
       }
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
+      for (int u = 0; u < U; ++u) {
         const float d = sqdist(r[u].x, r[u].y, r[u].z, qx, qy, qz);
         t.push(make_key(d, __float_as_uint(r[u].w)), s + u < len);
       }
-      s += kUnroll;
+      s += U;
     }
     if (!wave_any(s < len)) break;
     t.flush();
@@ -559,6 +579,31 @@ __device__ __forceinline__ void culled_column(const CloudInfo &ci,
 constexpr int kSlots = FK_SLOTS;
 constexpr int kStreamCols = 9;  // LDS table entries per lane
 
+// Candidate batch: the records a lane loads per round before the wave
+// checks whether any lane's key buffer could overflow.  A longer batch keeps
+// more loads in flight and checks less often, at 4 VGPRs per record and
+// less room left in the buffer (more merges); the best length depends on
+// the cloud (candidates per query, acceptance rate), so the first pass is
+// compiled for three: the K classes 8/16 take it in the stream scan
+// (default FK_SLOTS), 32/64 in the column scan (default FK_UNROLL), and
+// FK_QUERY_BATCH_SMALL / _LARGE select the alternatives (results never
+// change).
+enum BatchSel : int { kBatchDefault = 0, kBatchSmall = 1, kBatchLarge = 2 };
+template <int KC>
+constexpr int batch_records(int sel) {
+  return KC <= 16 ? (sel == kBatchSmall ? kSlots / 2
+                     : sel == kBatchLarge ? buffer_slots<KC>() * 3 / 4 : kSlots)
+                  : (sel == kBatchSmall ? kUnroll / 2
+                     : sel == kBatchLarge ? 2 * kUnroll : kUnroll);
+}
+template <int KC>
+constexpr int default_batch() { return batch_records<KC>(kBatchDefault); }
+// The column scan's batch in a kernel of candidate batch CB: CB for the K
+// classes that scan columns, the default for the stream classes (their
+// column scans serve only blocks wider than 27 cells).
+template <int KC, int CB>
+constexpr int range_batch() { return KC <= 16 ? kUnroll : CB; }
+
 // A record range [b, e) in one 64-bit LDS word of a key buffer, moved by
 // byte copies (so the words may hold keys at other times without any
 // type-based aliasing assumption).
@@ -581,7 +626,7 @@ constexpr int stream_entries() {
 
 // The ranges live in a lane-private LDS table (entry c at tab[c * kWave]):
 // in registers they would cost 18 VGPRs and a wave per SIMD.
-template <int NC, int G>
+template <int NC, int G, int S = kSlots>
 struct ColumnStream {
   uint64_t *tab;  // this lane's table: (first record of this lane, range end)
   uint32_t rest;  // non-empty ranges not yet started (bit c)
@@ -625,15 +670,15 @@ struct ColumnStream {
     return ok ? (uint32_t)(base + u * G) : 0u;
   }
 
-  // After a round of kSlots records: the current range continues, or the
+  // After a round of S records: the current range continues, or the
   // following one (partly taken) becomes current, or both are done.
   __device__ __forceinline__ void advance() {
-    if (lc > kSlots) {
-      pc += kSlots * G;
-      lc -= kSlots;
+    if (lc > S) {
+      pc += S * G;
+      lc -= S;
       return;
     }
-    const int used = kSlots - lc;
+    const int used = S - lc;
     if (ln > used) {
       pc = pn + used * G;
       lc = ln - used;
@@ -645,22 +690,22 @@ struct ColumnStream {
   }
 };
 
-template <int KC, int G, int NC>
-__device__ __forceinline__ void scan_stream(TopK<KC> &t, ColumnStream<NC, G> &cs,
+template <int KC, int G, int NC, int S>
+__device__ __forceinline__ void scan_stream(TopK<KC> &t, ColumnStream<NC, G, S> &cs,
                                             float4 qr,
                                             const float4 *__restrict__ srec) {
-  constexpr int kRoom = buffer_slots<KC>() - kSlots;
+  constexpr int kRoom = buffer_slots<KC>() - S;
   static_assert(kRoom >= 0, "buffer too small for a round");
   for (;;) {
     while (wave_any(cs.lc > 0) && !wave_any(t.cnt > kRoom)) {
-      float4 r[kSlots];
-      bool ok[kSlots];
+      float4 r[S];
+      bool ok[S];
 #pragma unroll
-      for (int u = 0; u < kSlots; ++u) {
+      for (int u = 0; u < S; ++u) {
         r[u] = srec[cs.addr(u, ok[u])];
       }
 #pragma unroll
-      for (int u = 0; u < kSlots; ++u) {
+      for (int u = 0; u < S; ++u) {
         const float d = sqdist(r[u].x, r[u].y, r[u].z, qr.x, qr.y, qr.z);
         t.push(make_key(d, __float_as_uint(r[u].w)), ok[u]);
       }
@@ -686,7 +731,7 @@ __device__ __forceinline__ void ring_columns(const QueryArgs &a, const CloudInfo
   }
 }
 
-template <int KC, int G>
+template <int KC, int G, int CB>
 __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
                                            const CloudInfo &ci, int4 qc,
                                            float4 qr, int R, bool active,
@@ -706,9 +751,9 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
       if (a.prune) {  // launch-uniform
         {
           const int b[1] = {b0}, e[1] = {e0};
-          ColumnStream<1, G> cs;
+          ColumnStream<1, G, CB> cs;
           cs.init(lane_tab, b, e, sub);
-          scan_stream<KC, G, 1>(t, cs, qr, a.srec);
+          scan_stream<KC, G, 1, CB>(t, cs, qr, a.srec);
         }
         if (wave_any(t.cnt > 0)) t.flush();
         if constexpr (G > 1) t.template group_merge_keep<G>(lane, sub);
@@ -716,9 +761,9 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
         cu.setup(ci, qc, qr, __uint_as_float(key_dbits(t.thr)));
         int b[8], e[8];
         ring_columns<8>(a, ci, qc, active, cu, b, e, 0);
-        ColumnStream<8, G> cs;
+        ColumnStream<8, G, CB> cs;
         cs.init(lane_tab, b, e, sub);
-        scan_stream<KC, G, 8>(t, cs, qr, a.srec);
+        scan_stream<KC, G, 8, CB>(t, cs, qr, a.srec);
       } else {
         Cull cu;
         cu.setup(ci, qc, qr, INFINITY);
@@ -726,9 +771,9 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
         b[0] = b0;
         e[0] = e0;
         ring_columns<9>(a, ci, qc, active, cu, b, e, 1);
-        ColumnStream<9, G> cs;
+        ColumnStream<9, G, CB> cs;
         cs.init(lane_tab, b, e, sub);
-        scan_stream<KC, G, 9>(t, cs, qr, a.srec);
+        scan_stream<KC, G, 9, CB>(t, cs, qr, a.srec);
       }
       return;
     }
@@ -737,7 +782,7 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
     int b = 0, e = 0;
     if (active)
       column_range(ci, a.cell_start, a.scell, qc.x, qc.y, qc.z - R, qc.z + R, b, e);
-    scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
+    scan_range<KC, G, range_batch<KC, CB>()>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
   }
   for (int dx = -R; dx <= R; ++dx) {
     for (int dy = -R; dy <= R; ++dy) {
@@ -746,7 +791,7 @@ __device__ __forceinline__ void scan_block(TopK<KC> &t, const QueryArgs &a,
       if (active)
         column_range(ci, a.cell_start, a.scell, qc.x + dx, qc.y + dy, qc.z - R,
                      qc.z + R, b, e);
-      scan_range<KC, G>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
+      scan_range<KC, G, range_batch<KC, CB>()>(t, b, e, sub, qr.x, qr.y, qr.z, a.srec);
     }
   }
 }
@@ -919,7 +964,7 @@ __device__ __forceinline__ void inline_expand(const QueryArgs &a, int q, int lan
 // batched kernel.  With INL (the plain kernel) it returns the sorted index of
 // a query left for the wave to expand (its list in the wave's seed area), or
 // -1.
-template <int KC, int G, bool INL = false>
+template <int KC, int G, bool INL = false, int CB = default_batch<KC>()>
 __device__ __forceinline__ int run_tile(const QueryArgs &a, int q0, int qend,
                                         uint64_t *lane_buf, uint64_t *lane_tab) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -933,7 +978,7 @@ __device__ __forceinline__ int run_tile(const QueryArgs &a, int q0, int qend,
   const int k = a.k_per_cloud ? a.k_per_cloud[qc.w] : a.k;
   TopK<KC> t;
   t.init(lane_buf, k, a.cut_r2, pad_value(a, qr, qs));
-  scan_block<KC, G>(t, a, ci, qc, qr, a.R, active, sub, lane, lane_tab);
+  scan_block<KC, G, CB>(t, a, ci, qc, qr, a.R, active, sub, lane, lane_tab);
   if (wave_any(t.cnt > 0)) t.flush();
   if constexpr (G > 1) t.template group_merge<G>(lane);
 
@@ -996,8 +1041,9 @@ constexpr int waves_per_block() {
   return KC <= 16 ? 4 : (KC == 32 ? 2 : 1);
 }
 
-// G lanes per query (the default per K class, or 1 with FK_QUERY_ONE_LANE).
-template <int KC, int G>
+// G lanes per query (the default per K class, or 1 with FK_QUERY_ONE_LANE),
+// CB records per lane per scan round.
+template <int KC, int G, int CB = default_batch<KC>()>
 __global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
 k_query(QueryArgs a) {
   constexpr int WPB = waves_per_block<KC>();
@@ -1012,7 +1058,7 @@ k_query(QueryArgs a) {
   // (its tables may then be incomplete)
   if (q0 >= a.nq || a.scalars[kStatus] != 0) return;
   uint64_t *wb = sbuf + w * BS * kWave, *wt = stab + w * stream_entries<KC>() * kWave;
-  const int qx = run_tile<KC, G, (FK_INLINE_PLAIN != 0)>(a, q0, a.nq, wb + lane, wt + lane);
+  const int qx = run_tile<KC, G, (FK_INLINE_PLAIN != 0), CB>(a, q0, a.nq, wb + lane, wt + lane);
   if constexpr (FK_INLINE_PLAIN != 0)
     if (qx >= 0) inline_expand<KC>(a, qx, lane, wb, wt);
 }
@@ -1021,7 +1067,7 @@ k_query(QueryArgs a) {
 // lanes per query (see launch_plan).  One kernel serves every G, so a
 // launch mixes wide groups for the dense super-tiles with narrow ones for
 // the sparse.
-template <int KC>
+template <int KC, int CB = default_batch<KC>()>
 __global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
 k_query_planned(QueryArgs a) {
   constexpr int WPB = waves_per_block<KC>();
@@ -1041,10 +1087,10 @@ k_query_planned(QueryArgs a) {
   const int qend = min(a.nq, q0 + kWave / g);
   int qx;
   switch (g) {
-    case 1: qx = run_tile<KC, 1, true>(a, q0, qend, wb + lane, wt + lane); break;
-    case 2: qx = run_tile<KC, 2, true>(a, q0, qend, wb + lane, wt + lane); break;
-    case 4: qx = run_tile<KC, 4, true>(a, q0, qend, wb + lane, wt + lane); break;
-    default: qx = run_tile<KC, 8, true>(a, q0, qend, wb + lane, wt + lane); break;
+    case 1: qx = run_tile<KC, 1, true, CB>(a, q0, qend, wb + lane, wt + lane); break;
+    case 2: qx = run_tile<KC, 2, true, CB>(a, q0, qend, wb + lane, wt + lane); break;
+    case 4: qx = run_tile<KC, 4, true, CB>(a, q0, qend, wb + lane, wt + lane); break;
+    default: qx = run_tile<KC, 8, true, CB>(a, q0, qend, wb + lane, wt + lane); break;
   }
   if (qx >= 0) inline_expand<KC>(a, qx, lane, wb, wt);
 }
@@ -1328,6 +1374,7 @@ __device__ __forceinline__ int coop_expand(CoopList<P> &t, const QueryArgs &a,
   }
 }
 
+#if FK_EXPAND_PART
 __global__ void __launch_bounds__(64 * kCoopWaves, 4) k_expand_coop(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   __shared__ uint64_t ctab[kCoopWaves * kShellTab];
@@ -1365,6 +1412,7 @@ __global__ void __launch_bounds__(64 * kCoopWaves, 4) k_expand_coop(QueryArgs a)
   }
   if (lane == 0 && nexp) atomicAdd(a.qcnt + kCntExpanded, nexp);
 }
+#endif  // FK_COMMON
 
 // The tile's LDS (key buffers, stream table) once its scan is done: the
 // shell table at the start of the key buffers, then the list buffer and the
@@ -1422,6 +1470,7 @@ __device__ __forceinline__ void inline_expand(const QueryArgs &a, int q, int lan
 // k in (64, 128]: one wave per query from the start, the best list held as
 // two keys per lane.  Scans the block of half-width a.R; FK_MODE_EXACT then
 // grows it until the query certifies.
+#if FK_WIDE_PART
 __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
   __shared__ uint64_t sbuf[kCoopWaves * kCoopCap];
   __shared__ uint64_t ctab[kCoopWaves * kShellTab];
@@ -1461,6 +1510,7 @@ __global__ void __launch_bounds__(64 * kCoopWaves) k_query_wide(QueryArgs a) {
     atomicAdd(a.qcnt + kCntExpanded, nexp);
   }
 }
+#endif  // FK_COMMON
 
 // ----------------------------------------------------- batched persistent
 
@@ -1476,45 +1526,41 @@ __device__ __forceinline__ int item_to_cloud(const int32_t *prefix, int B,
   return lo;
 }
 
-// Largest LDS buffer of the K classes up to MAXKC.
-template <int MAXKC>
-constexpr int batched_slots() {
-  int m = buffer_slots<8>();
-  if (MAXKC >= 16 && buffer_slots<16>() > m) m = buffer_slots<16>();
-  if (MAXKC >= 32 && buffer_slots<32>() > m) m = buffer_slots<32>();
-  if (MAXKC >= 64 && buffer_slots<64>() > m) m = buffer_slots<64>();
-  return m;
+// The batched launch's K classes (k up to 8, 16, 32, 64 -> 0..3) and
+// their work items (kWave / G queries of one cloud).
+__host__ __device__ constexpr int batched_class(int k) {
+  return k <= 8 ? 0 : (k <= 16 ? 1 : (k <= 32 ? 2 : 3));
 }
-
-// Lanes per query of a batched launch (one value for all its K classes, so
-// that every work item is one tile of kWave / G queries).
-template <int MAXKC>
-constexpr int batched_lanes() {
-  return MAXKC <= 32 ? 2 : 1;
+__host__ __device__ constexpr int batched_tile(int cls) {
+  return kWave / (cls == 0   ? lanes_per_query<8>()
+                  : cls == 1 ? lanes_per_query<16>()
+                  : cls == 2 ? lanes_per_query<32>()
+                             : lanes_per_query<64>());
 }
+static_assert(batched_class(64) == kBatchedClasses - 1, "batched classes");
 
-template <int MAXKC>
-__global__ void __launch_bounds__(64 * waves_per_block<MAXKC>())
-k_batched(QueryArgs a) {
-  constexpr int WPB = waves_per_block<MAXKC>();
-  constexpr int BS = batched_slots<MAXKC>();
-  constexpr int G = batched_lanes<MAXKC>();
-  // The column table of the K class 8/16 tiles: the tail of the lane's
-  // buffer when it is long enough (those tiles use its first 16 slots; the
-  // K class 32/64 tiles, which use all of it, take no table), else its own
-  // array.  (A separate table cut the K class 32 kernel to three blocks per
-  // CU.)
-  constexpr bool kTabInBuf = BS >= 16 + kStreamCols;
+// Batched first pass of one K class: a persistent kernel whose waves pull
+// (cloud, tile) work items of the clouds of that class from the class's own
+// queue (tiles of kWave / G queries).  One kernel per class keeps each
+// class's register budget and occupancy: a single kernel for every class
+// ran all tiles at the budget of the largest (two waves per SIMD).
+template <int KC>
+__global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
+k_batched(QueryArgs a, int cls) {
+  constexpr int WPB = waves_per_block<KC>();
+  constexpr int BS = buffer_slots<KC>();
+  constexpr int G = lanes_per_query<KC>();
   __shared__ uint64_t sbuf[WPB * BS * kWave];
-  __shared__ uint64_t stab[kTabInBuf ? 1 : WPB * kStreamCols * kWave];
+  __shared__ uint64_t stab[WPB * stream_entries<KC>() * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   uint64_t *lane_buf = sbuf + w * BS * kWave + lane;
-  uint64_t *lane_tab = kTabInBuf ? lane_buf + 16 * kWave : stab + w * kStreamCols * kWave + lane;
-  const int32_t *prefix = a.item_cloud;
+  uint64_t *lane_tab = stab + w * stream_entries<KC>() * kWave + lane;
+  const int32_t *prefix = a.item_cloud + cls * (a.num_clouds + 1);
+  int32_t *head = a.scalars + kQueueHead + cls;
   const int nitems = a.scalars[kStatus] != 0 ? 0 : prefix[a.num_clouds];
   for (;;) {
     int item = 0;
-    if (lane == 0) item = atomicAdd(a.scalars + kQueueHead, 1);
+    if (lane == 0) item = atomicAdd(head, 1);
     item = __shfl(item, 0, kWave);
     if (item >= nitems) break;
     const int c = item_to_cloud(prefix, a.num_clouds, item);
@@ -1522,31 +1568,21 @@ k_batched(QueryArgs a) {
     const int qb = a.cell_start[ci.base];
     const int qe = a.cell_start[ci.base + table_entries(ci)];
     const int q0 = qb + (item - prefix[c]) * (kWave / G);
-    const int k = a.k_per_cloud[c];
-    if (k < 1 || k > a.k) {
-      // a per-cloud k above the launch's k class: refuse the tile
-      if (lane == 0) raise_status(a.scalars, FK_ERR_K_RANGE);
-      continue;
-    }
-    if (k <= 8 || MAXKC == 8)
-      run_tile<8, G>(a, q0, qe, lane_buf, lane_tab);
-    else if (k <= 16 || MAXKC == 16)
-      run_tile<(MAXKC < 16 ? MAXKC : 16), G>(a, q0, qe, lane_buf, lane_tab);
-    else if (k <= 32 || MAXKC == 32)
-      run_tile<(MAXKC < 32 ? MAXKC : 32), G>(a, q0, qe, lane_buf, lane_tab);
-    else
-      run_tile<MAXKC, G>(a, q0, qe, lane_buf, lane_tab);
+    run_tile<KC, G>(a, q0, qe, lane_buf, lane_tab);
   }
 }
 
-#if !FK_COLUMNS
-// Per-cloud tile counts -> exclusive prefix (one block).
+#if !FK_COLUMNS && FK_COMMON
+// Per-class tile counts -> one exclusive prefix per K class over the clouds
+// (one block): prefix[cls * (B + 1) + c] is cloud c's first item in class
+// cls's queue (clouds of other classes contribute none).  A cloud whose k
+// lies outside [1, max_k] raises FK_ERR_K_RANGE, and then no tile runs.
 __global__ void __launch_bounds__(1024)
-k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
-        int32_t tile, int32_t *prefix, int32_t *scalars, int32_t *qcnt) {
-  __shared__ int wsum[16];
-  __shared__ int carry;
-  if (threadIdx.x == 0) carry = 0;
+k_items(const CloudInfo *clouds, const int32_t *cell_start, const int32_t *k_per_cloud,
+        int32_t B, int32_t max_k, int32_t *prefix, int32_t *scalars, int32_t *qcnt) {
+  __shared__ int wsum[kBatchedClasses][16];
+  __shared__ int carry[kBatchedClasses];
+  if (threadIdx.x < kBatchedClasses) carry[threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // After a build that raised an error (an overflowing no-sync build still
@@ -1554,30 +1590,50 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, int32_t B,
   const bool ok = scalars[kStatus] == 0;
   for (int c0 = 0; c0 < B; c0 += 1024) {
     const int c = c0 + threadIdx.x;
-    int tiles = 0;
+    int tiles = 0, cls = -1;
     if (ok && c < B) {
-      const CloudInfo ci = clouds[c];
-      const int n = cell_start[ci.base + table_entries(ci)] - cell_start[ci.base];
-      tiles = (n + tile - 1) / tile;
+      const int k = k_per_cloud[c];
+      if (k < 1 || k > max_k) {
+        raise_status(scalars, FK_ERR_K_RANGE);
+      } else {
+        const CloudInfo ci = clouds[c];
+        const int n = cell_start[ci.base + table_entries(ci)] - cell_start[ci.base];
+        cls = batched_class(k);
+        const int tile = batched_tile(cls);
+        tiles = (n + tile - 1) / tile;
+      }
     }
-    int v = tiles;
+    int v[kBatchedClasses];
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const int u = __shfl_up(v, o, kWave);
-      if (lane >= o) v += u;
+    for (int j = 0; j < kBatchedClasses; ++j) {
+      v[j] = cls == j ? tiles : 0;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v[j], o, kWave);
+        if (lane >= o) v[j] += u;
+      }
+      if (lane == 63) wsum[j][w] = v[j];
     }
-    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    int before = carry;
-    for (int j = 0; j < w; ++j) before += wsum[j];
-    if (c < B) prefix[c] = before + v - tiles;
+#pragma unroll
+    for (int j = 0; j < kBatchedClasses; ++j) {
+      int before = carry[j];
+      for (int i = 0; i < w; ++i) before += wsum[j][i];
+      if (c < B) prefix[j * (B + 1) + c] = before + v[j] - (cls == j ? tiles : 0);
+      v[j] += before;
+    }
     __syncthreads();
-    if (threadIdx.x == 1023) carry = before + v;
+    if (threadIdx.x == 1023) {
+#pragma unroll
+      for (int j = 0; j < kBatchedClasses; ++j) carry[j] = v[j];
+    }
     __syncthreads();
   }
+  if (threadIdx.x < kBatchedClasses) {
+    prefix[threadIdx.x * (B + 1) + B] = carry[threadIdx.x];
+    scalars[kQueueHead + threadIdx.x] = 0;
+  }
   if (threadIdx.x == 0) {
-    prefix[B] = carry;
-    scalars[kQueueHead] = 0;
     qcnt[kCntPending] = 0;
     qcnt[kCntExpanded] = 0;
   }
@@ -1607,6 +1663,7 @@ __device__ __forceinline__ unsigned long long query_candidates(const QueryArgs &
   return c;
 }
 
+#if FK_COMMON
 __global__ void __launch_bounds__(kCountBlock)
 k_candidate_count(const QueryArgs a, unsigned long long *total) {
   __shared__ unsigned long long wsum[kCountBlock / 64];
@@ -1624,9 +1681,11 @@ k_candidate_count(const QueryArgs a, unsigned long long *total) {
     if (t) atomicAdd(total, t);
   }
 }
+#endif  // FK_COMMON
 
 // Largest block candidate count of every super-tile of kPlanTile = kWave
 // consecutive queries (one wave per super-tile).
+#if FK_COMMON
 __global__ void __launch_bounds__(256) k_tile_cost(const QueryArgs a, int32_t *tile_cost) {
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
   int c = 0;
@@ -1639,6 +1698,7 @@ __global__ void __launch_bounds__(256) k_tile_cost(const QueryArgs a, int32_t *t
   for (int o = 32; o > 0; o >>= 1) c = max(c, __shfl_xor(c, o, kWave));
   if ((threadIdx.x & (kWave - 1)) == 0 && qi < a.nq) tile_cost[qi / kPlanTile] = c;
 }
+#endif  // FK_COMMON
 
 constexpr int kPlanMaxTiles = 8192;  // super-tiles cached in LDS by k_plan
 
@@ -1670,6 +1730,7 @@ __device__ __forceinline__ float plan_budget(int j) { return 4.f * exp2f(0.125f 
 // per-wave plan, super-tiles with more lanes first (they are the longest
 // waves: started first, they finish with the rest).  Unused plan entries
 // get (nq, 1) and exit at once.
+#if FK_COMMON
 __global__ void __launch_bounds__(1024)
 k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
        const int32_t *scalars, int2 *plan) {
@@ -1754,8 +1815,10 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
   for (int c = 0; c < 4; ++c) used += cls_n[c] * (8 >> c);
   for (int i = used + t; i < capacity; i += 1024) plan[i] = make_int2(nq, 1);
 }
+#endif  // FK_COMMON
 
 // Candidate count |C(q)| of every query, written to its output row.
+#if FK_COMMON
 __global__ void k_point_candidates(const QueryArgs a, int32_t *out) {
   const int qi = blockIdx.x * blockDim.x + threadIdx.x;
   if (qi >= a.nq || a.scalars[kStatus] != 0) return;
@@ -1771,11 +1834,13 @@ __global__ void k_point_candidates(const QueryArgs a, int32_t *out) {
     }
   out[__float_as_int(a.qrec[qi].w)] = (int32_t)(c < INT_MAX ? c : INT_MAX);
 }
+#endif  // FK_COMMON
 
 // Per-cloud candidate sums (self queries, block half-width a.R).  A block
 // whose queries all lie in one cloud (queries are sorted by cloud, so all
 // but the blocks at cloud boundaries) adds once; the others per wave, or per
 // lane where a wave spans clouds.
+#if FK_COMMON
 __global__ void __launch_bounds__(kCountBlock)
 k_cand_per_cloud(const QueryArgs a, unsigned long long *acc) {
   __shared__ unsigned long long wsum[kCountBlock / 64];
@@ -1822,9 +1887,11 @@ k_cand_per_cloud(const QueryArgs a, unsigned long long *acc) {
     if (cur >= 0 && run) atomicAdd(acc + cur, run);
   }
 }
+#endif  // FK_COMMON
 
 // side *= clamp((target_ratio * k / mean candidates)^(1/exponent)), bounded
 // so that the rebuilt cell box stays below ~64 cells per point.
+#if FK_COMMON
 __global__ void k_adapt(const CloudInfo *clouds, int32_t B,
                         const unsigned long long *acc, const int32_t *kpc,
                         int32_t k, float target_ratio, const float *ratio_pc,
@@ -1845,15 +1912,20 @@ __global__ void k_adapt(const CloudInfo *clouds, int32_t B,
   side[c] = s;
   inv[c] = (float)(1.0 / (double)s);
 }
+#endif  // FK_COMMON
 
+#if FK_COMMON
 __global__ void k_zero_u64(unsigned long long *p, int32_t n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
 }
+#endif  // FK_COMMON
 
+#if FK_COMMON
 __global__ void k_reset_query_counters(int32_t *qcnt) {
   if (threadIdx.x < kCntStride) qcnt[threadIdx.x] = 0;
 }
+#endif  // FK_COMMON
 
 // Fixed grid (graph-capturable): pending queries are read on the device.
 // Up to kExpandWaves waves, static striding over the pending list.  (One
@@ -1864,19 +1936,39 @@ __global__ void k_reset_query_counters(int32_t *qcnt) {
 #endif
 constexpr int kExpandWaves = FK_EXPAND_WAVES;
 
-unsigned expand_blocks(int nq) {
+[[maybe_unused]] unsigned expand_blocks(int nq) {
   const int waves = nq < kExpandWaves ? nq : kExpandWaves;
   const int b = (waves + kCoopWaves - 1) / kCoopWaves;
   return (unsigned)(b < 1 ? 1 : b);
 }
 
-template <int KC, int G>
+template <int KC, int G, int CB = default_batch<KC>()>
 void launch_first(const QueryArgs &a, hipStream_t s) {
   constexpr int WPB = waves_per_block<KC>();
   constexpr int QPW = kWave / G;  // queries per wave
   const int64_t waves = ((int64_t)a.nq + QPW - 1) / QPW;
   const unsigned nb = (unsigned)((waves + WPB - 1) / WPB);
-  hipLaunchKernelGGL((k_query<KC, G>), dim3(nb), dim3(64 * WPB), 0, s, a);
+  hipLaunchKernelGGL((k_query<KC, G, CB>), dim3(nb), dim3(64 * WPB), 0, s, a);
+}
+
+// f(std::integral_constant<int, CB>) for the candidate batch that sel
+// selects.  The alternatives exist for the K classes 8-32 of the dense
+// layout; the column layout and K class 64 keep the default.
+template <int KC, class F>
+void with_batch(int sel, F &&f) {
+#if !FK_COLUMNS
+  if constexpr (KC <= 32) {
+    if (sel == kBatchSmall) {
+      f(std::integral_constant<int, batch_records<KC>(kBatchSmall)>());
+      return;
+    }
+    if (sel == kBatchLarge) {
+      f(std::integral_constant<int, batch_records<KC>(kBatchLarge)>());
+      return;
+    }
+  }
+#endif
+  f(std::integral_constant<int, default_batch<KC>()>());
 }
 
 // A launch of few queries is bound by the latency of its longest scans, not
@@ -1889,63 +1981,111 @@ void launch_first(const QueryArgs &a, hipStream_t s) {
 #endif
 constexpr int64_t kFillWaves = FK_FILL_WAVES;
 
-template <int KC, int G>
+template <int KC, int G, int CB>
 void launch_grouped(const QueryArgs &a, hipStream_t s) {
   const int64_t lanes = (int64_t)a.nq * G;
   if constexpr (G < 16 && buffer_slots<KC>() >= KC) {
     if (lanes < kFillWaves * kWave) {
-      launch_grouped<KC, 2 * G>(a, s);
+      launch_grouped<KC, 2 * G, CB>(a, s);
       return;
     }
   }
-  launch_first<KC, G>(a, s);
+  launch_first<KC, G, CB>(a, s);
 }
 
+// First pass of K class KC: the balanced plan when the launch has one,
+// else uniform tiles (one lane per query with FK_QUERY_ONE_LANE, widened
+// groups for small launches).
 template <int KC>
-hipError_t launch_plain(const QueryArgs &a, hipStream_t s) {
+void launch_plain(const QueryArgs &a, hipStream_t s) {
   constexpr int G = lanes_per_query<KC>();
-  if (a.passes & 2) {
-    if constexpr (KC <= 32) {
-      if (a.plan) {
-        constexpr int WPB = waves_per_block<KC>();
-        hipLaunchKernelGGL(k_query_planned<KC>, dim3(a.plan_waves / WPB), dim3(64 * WPB),
-                           0, s, a);
-        goto expand;
-      }
+  if constexpr (KC <= 32) {
+    if (a.plan) {
+      constexpr int WPB = waves_per_block<KC>();
+      with_batch<KC>(a.batch_sel, [&](auto cb) {
+        hipLaunchKernelGGL((k_query_planned<KC, decltype(cb)::value>),
+                           dim3(a.plan_waves / WPB), dim3(64 * WPB), 0, s, a);
+      });
+      return;
     }
-    if (G > 1 && a.one_lane)
-      launch_first<KC, 1>(a, s);
-    else if constexpr (G > 1)
-      launch_grouped<KC, G>(a, s);
-    else
-      launch_first<KC, G>(a, s);
   }
-expand:
-  if (a.exact && (a.passes & 4))
-    hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)),
-                       dim3(64 * kCoopWaves), 0, s, a);
-  return hipGetLastError();
-}
-
-template <int MAXKC>
-hipError_t launch_batched_t(const QueryArgs &a, hipStream_t s) {
-  constexpr int WPB = waves_per_block<MAXKC>();
-  int dev = 0, cus = 256;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess) return e;
-  // persistent: a few resident workgroups per CU pull tiles from the queue
-  const int per_cu = MAXKC <= 16 ? 4 : 5;
-  unsigned nb = (unsigned)(cus * per_cu);
-  hipLaunchKernelGGL(k_batched<MAXKC>, dim3(nb), dim3(64 * WPB), 0, s, a);
-  if (a.exact)
-    hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)),
-                       dim3(64 * kCoopWaves), 0, s, a);
-  return hipGetLastError();
+  if (G > 1 && a.one_lane) {
+    with_batch<KC>(a.batch_sel,
+                   [&](auto cb) { launch_first<KC, 1, decltype(cb)::value>(a, s); });
+  } else if constexpr (G > 1) {
+    with_batch<KC>(a.batch_sel,
+                   [&](auto cb) { launch_grouped<KC, G, decltype(cb)::value>(a, s); });
+  } else {
+    launch_first<KC, G>(a, s);
+  }
 }
 
 }  // namespace
+
+// ------------------------------------------------- per K class launchers
+// (defined and explicitly instantiated only in the unit of their class: the
+// other units see the declarations and do not compile the class's kernels)
+#if FK_QPART == 0 || FK_QPART >= 8
+
+template <int KC>
+hipError_t FK_LAYOUT(launch_first_pass)(const QueryArgs &a, hipStream_t s) {
+  launch_plain<KC>(a, s);
+  return hipGetLastError();
+}
+
+// Persistent: as many resident workgroups as fit pull the class's tiles.
+template <int KC>
+hipError_t FK_LAYOUT(launch_batched_class)(const QueryArgs &a, int cls, hipStream_t s) {
+  constexpr int WPB = waves_per_block<KC>();
+  int dev = 0, cus = 0, nb = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched<KC>, 64 * WPB, 0);
+  if (e != hipSuccess) return e;
+  const unsigned grid = (unsigned)((nb < 1 ? 1 : nb) * cus);
+  hipLaunchKernelGGL(k_batched<KC>, dim3(grid), dim3(64 * WPB), 0, s, a, cls);
+  return hipGetLastError();
+}
+
+#if !FK_COLUMNS
+template <int KC>
+hipError_t planned_blocks(int sel, int *nb) {
+  hipError_t e = hipSuccess;
+  with_batch<KC>(sel, [&](auto cb) {
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+        nb, k_query_planned<KC, decltype(cb)::value>, 64 * waves_per_block<KC>(), 0);
+  });
+  return e;
+}
+#define FK_PLANNED_BLOCKS(KC) template hipError_t planned_blocks<KC>(int, int *);
+#else
+#define FK_PLANNED_BLOCKS(KC)
+#endif
+
+#define FK_CLASS_UNIT(KC)                                                        \
+  template hipError_t FK_LAYOUT(launch_first_pass)<KC>(const QueryArgs &,        \
+                                                       hipStream_t);             \
+  template hipError_t FK_LAYOUT(launch_batched_class)<KC>(const QueryArgs &, int, \
+                                                          hipStream_t);          \
+  FK_PLANNED_BLOCKS(KC)
+#if FK_CLASS(8)
+FK_CLASS_UNIT(8)
+#endif
+#if FK_CLASS(16)
+FK_CLASS_UNIT(16)
+#endif
+#if FK_CLASS(32)
+FK_CLASS_UNIT(32)
+#endif
+#if FK_CLASS(64)
+template hipError_t FK_LAYOUT(launch_first_pass)<64>(const QueryArgs &, hipStream_t);
+template hipError_t FK_LAYOUT(launch_batched_class)<64>(const QueryArgs &, int, hipStream_t);
+#endif
+#endif  // class units
+
+// ------------------------------------------------------- dispatching launchers
+#if FK_COMMON
 
 #if !FK_COLUMNS
 int kclass_of(int k) {
@@ -1963,19 +2103,21 @@ hipError_t FK_LAYOUT(launch_query)(const QueryArgs &a, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_query_counters, dim3(1), dim3(64), 0, s, a.qcnt);
   if (a.nq <= 0) return hipGetLastError();
   if (a.k > kMaxLaneK) {
-    if (a.passes & 2) {
-      const int64_t b = ((int64_t)a.nq + kCoopWaves - 1) / kCoopWaves;
-      const unsigned nb = (unsigned)(b < 16384 ? b : 16384);
-      hipLaunchKernelGGL(k_query_wide, dim3(nb), dim3(64 * kCoopWaves), 0, s, a);
-    }
+    if (a.passes & 2) return FK_LAYOUT(launch_wide)(a, s);
     return hipGetLastError();
   }
-  switch (kclass_of(a.k)) {
-    case 8: return launch_plain<8>(a, s);
-    case 16: return launch_plain<16>(a, s);
-    case 32: return launch_plain<32>(a, s);
-    default: return launch_plain<64>(a, s);
+  if (a.passes & 2) {
+    hipError_t e;
+    switch (kclass_of(a.k)) {
+      case 8: e = FK_LAYOUT(launch_first_pass)<8>(a, s); break;
+      case 16: e = FK_LAYOUT(launch_first_pass)<16>(a, s); break;
+      case 32: e = FK_LAYOUT(launch_first_pass)<32>(a, s); break;
+      default: e = FK_LAYOUT(launch_first_pass)<64>(a, s); break;
+    }
+    if (e != hipSuccess) return e;
   }
+  if (a.exact && (a.passes & 4)) return FK_LAYOUT(launch_expand)(a, s);
+  return hipGetLastError();
 }
 
 hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
@@ -1998,7 +2140,7 @@ hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
 #endif
 constexpr int kPlanOver = FK_PLAN_OVER;
 
-hipError_t planned_capacity(int kc, int32_t *waves) {
+hipError_t planned_capacity(int kc, int batch_sel, int32_t *waves) {
   int dev = 0, cus = 0, nb = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -2006,24 +2148,24 @@ hipError_t planned_capacity(int kc, int32_t *waves) {
   int wpb = 0;
   switch (kc) {
     case 8:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_planned<8>,
-                                                       64 * waves_per_block<8>(), 0);
-      wpb = waves_per_block<8>();
+      e = planned_blocks<8>(batch_sel, &nb);
+      wpb = 4;
       break;
     case 16:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_planned<16>,
-                                                       64 * waves_per_block<16>(), 0);
-      wpb = waves_per_block<16>();
+      e = planned_blocks<16>(batch_sel, &nb);
+      wpb = 4;
       break;
     case 32:
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_planned<32>,
-                                                       64 * waves_per_block<32>(), 0);
-      wpb = waves_per_block<32>();
+      e = planned_blocks<32>(batch_sel, &nb);
+      wpb = 2;
       break;
     default:
       return hipErrorInvalidValue;
   }
   if (e != hipSuccess) return e;
+  static_assert(waves_per_block<8>() == 4 && waves_per_block<16>() == 4 &&
+                    waves_per_block<32>() == 2,
+                "planned capacity: waves per block");
   // the plan may exceed one resident round by kPlanOver percent: the
   // lightest super-tiles, planned last, start when the first waves retire
   const int64_t resident = (int64_t)(nb < 1 ? 1 : nb) * cus * wpb;
@@ -2082,34 +2224,45 @@ hipError_t FK_LAYOUT(launch_adapt_cell)(const QueryArgs &a, unsigned long long *
 }
 
 #if !FK_COLUMNS
-int batched_tile(int32_t max_k) {
-  switch (kclass_of(max_k)) {
-    case 8: return kWave / batched_lanes<8>();
-    case 16: return kWave / batched_lanes<16>();
-    case 32: return kWave / batched_lanes<32>();
-    default: return kWave / batched_lanes<64>();
-  }
-}
-
 hipError_t launch_items(const CloudInfo *clouds, const int32_t *cell_start,
-                        int32_t B, int32_t max_k, int32_t *prefix,
-                        int32_t *scalars, int32_t *qcnt, hipStream_t s) {
-  hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, s, clouds, cell_start, B,
-                     batched_tile(max_k), prefix, scalars, qcnt);
+                        const int32_t *k_per_cloud, int32_t B, int32_t max_k,
+                        int32_t *prefix, int32_t *scalars, int32_t *qcnt,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_items, dim3(1), dim3(1024), 0, s, clouds, cell_start, k_per_cloud,
+                     B, max_k, prefix, scalars, qcnt);
   return hipGetLastError();
 }
-
 #endif  // !FK_COLUMNS
 
-hipError_t FK_LAYOUT(launch_batched)(const QueryArgs &a, int32_t max_k,
-                                     hipStream_t s) {
-  FK_FORWARD(launch_batched_columns(a, max_k, s));
-  switch (kclass_of(max_k)) {
-    case 8: return launch_batched_t<8>(a, s);
-    case 16: return launch_batched_t<16>(a, s);
-    case 32: return launch_batched_t<32>(a, s);
-    default: return launch_batched_t<64>(a, s);
+hipError_t FK_LAYOUT(launch_batched_first)(const QueryArgs &a, int cls, hipStream_t s) {
+  FK_FORWARD(launch_batched_first_columns(a, cls, s));
+  switch (cls) {
+    case 0: return FK_LAYOUT(launch_batched_class)<8>(a, cls, s);
+    case 1: return FK_LAYOUT(launch_batched_class)<16>(a, cls, s);
+    case 2: return FK_LAYOUT(launch_batched_class)<32>(a, cls, s);
+    case 3: return FK_LAYOUT(launch_batched_class)<64>(a, cls, s);
+    default: return hipErrorInvalidValue;
   }
 }
+
+#endif  // FK_COMMON
+
+#if FK_EXPAND_PART
+hipError_t FK_LAYOUT(launch_expand)(const QueryArgs &a, hipStream_t s) {
+  if (a.exact && a.nq > 0)
+    hipLaunchKernelGGL(k_expand_coop, dim3(expand_blocks(a.nq)), dim3(64 * kCoopWaves), 0,
+                       s, a);
+  return hipGetLastError();
+}
+#endif
+
+#if FK_WIDE_PART
+hipError_t FK_LAYOUT(launch_wide)(const QueryArgs &a, hipStream_t s) {
+  const int64_t b = ((int64_t)a.nq + kCoopWaves - 1) / kCoopWaves;
+  const unsigned nb = (unsigned)(b < 16384 ? b : 16384);
+  hipLaunchKernelGGL(k_query_wide, dim3(nb), dim3(64 * kCoopWaves), 0, s, a);
+  return hipGetLastError();
+}
+#endif
 
 }  // namespace fk

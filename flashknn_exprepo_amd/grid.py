@@ -19,6 +19,10 @@ def _dev(t: torch.Tensor, dtype: torch.dtype, name: str) -> torch.Tensor:
     return t.to(dtype).contiguous()
 
 
+# candidate batch of the first pass -> fk_query_params.flags
+BATCH_FLAGS = {"small": F.FK_QUERY_BATCH_SMALL, "default": 0, "large": F.FK_QUERY_BATCH_LARGE}
+
+
 def _build_flags(no_sync: bool, columns: bool) -> int:
     return (F.FK_BUILD_NO_SYNC if no_sync else 0) | (F.FK_BUILD_COLUMNS if columns else 0)
 
@@ -27,9 +31,11 @@ class Grid:
     """Cell-sorted support points of one or more clouds on one HIP device."""
 
     # speed hints for exact queries of the last build_exact_grid
-    # (FK_QUERY_PRUNE, FK_QUERY_ONE_LANE), chosen by the autotuner
+    # (FK_QUERY_PRUNE, FK_QUERY_ONE_LANE, the candidate batch), chosen by the
+    # autotuner
     prune_hint = False
     one_lane_hint = False
+    batch_hint = "default"
 
     def __init__(self):
         self._L = F.load()
@@ -41,6 +47,12 @@ class Grid:
         self.num_points = 0
         self.num_clouds = 0
         self.num_queries = -1
+
+    def hints(self):
+        """The autotuned speed hints of the last build_exact_grid as keyword
+        arguments of query() (results never depend on them)."""
+        return dict(prune=self.prune_hint, one_lane=self.one_lane_hint,
+                    batch=self.batch_hint)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -195,11 +207,14 @@ class Grid:
     def query(self, k, *, mode=F.FK_MODE_CELLS, radius_cells=1, cut_r2=math.inf,
               idx_dtype=torch.int64, pad_mode=F.FK_PAD_SELF, dist_mode=F.FK_DIST_SQ,
               pad_dist=math.inf, attached=False, out_idx=None, out_dist=None,
-              passes=0, prune=False, one_lane=False, uniform=False):
+              passes=0, prune=False, one_lane=False, uniform=False, batch="default"):
         """passes: 0 = full query; else a mask (1 reset, 2 first pass,
         4 expansion) used to time one kernel alone.  prune / one_lane /
-        uniform: FK_QUERY_PRUNE / FK_QUERY_ONE_LANE / FK_QUERY_UNIFORM
-        (speed hints; results are identical)."""
+        uniform: FK_QUERY_PRUNE / FK_QUERY_ONE_LANE / FK_QUERY_UNIFORM;
+        batch: "small" / "default" / "large" candidate batch
+        (FK_QUERY_BATCH_SMALL / _LARGE).  Speed hints: results are identical."""
+        if batch not in BATCH_FLAGS:
+            raise ValueError(f"batch must be one of {sorted(BATCH_FLAGS)}")
         nq = self.num_queries if attached else self.num_points
         if attached and nq < 0:
             raise RuntimeError("no queries attached")
@@ -213,7 +228,7 @@ class Grid:
                           F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
                           passes, (F.FK_QUERY_PRUNE if prune else 0)
                           | (F.FK_QUERY_ONE_LANE if one_lane else 0)
-                          | (F.FK_QUERY_UNIFORM if uniform else 0))
+                          | (F.FK_QUERY_UNIFORM if uniform else 0) | BATCH_FLAGS[batch])
         F.check(self._L.fk_query(self._h, ctypes.byref(p), F.ptr(out_idx),
                                  F.ptr(out_dist), F.stream_ptr()), "fk_query")
         return out_idx, out_dist

@@ -99,7 +99,7 @@ def density_class(mean_candidates, k) -> int:
 
 def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
                      target_ratio=None, adapt=True, rounds=None, no_sync=True,
-                     prune=None, skip_band=None):
+                     prune=None, skip_band=None, min_points=ADAPT_MIN_POINTS):
     """Build ``g`` for exact queries of (xyz, offsets); returns the per-cloud
     cell side.  First guess from the bounding-box density, then (clouds of
     >= ADAPT_MIN_POINTS points) density-feedback rebuilds towards the
@@ -109,10 +109,13 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
     rebuilds are skipped when every cloud's first-guess candidates per query
     lie within a factor b of its target.  Sets ``g.prune_hint``
     (FK_QUERY_PRUNE for the queries on this build): ``prune`` if given, else
-    the autotuner's choice for the clouds holding most points."""
+    the autotuner's choice for the clouds holding most points; likewise
+    ``g.one_lane_hint`` and ``g.batch_hint`` (the candidate batch).  Builds of
+    fewer than ``min_points`` points keep the first guess."""
     n = xyz.shape[0]
     g.prune_hint = bool(prune) if prune is not None else False
     g.one_lane_hint = False
+    g.batch_hint = "default"
     if cell is not None:
         side = torch.as_tensor(cell, dtype=torch.float32, device=xyz.device).reshape(-1)
         side = side.expand(offsets.shape[0]).contiguous()
@@ -120,10 +123,11 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
         return side
     side, inv = g.auto_cell(xyz, offsets, k, DEFAULT_CELL_SCALE if scale is None else scale)
     g.build_xyz(xyz, offsets, inv, no_sync=no_sync)
-    if not adapt or n < ADAPT_MIN_POINTS:
+    if not adapt or n < min_points:
         return side
     if target_ratio is None:
-        from .autotune import lookup_one_lane, lookup_prune, lookup_target_ratio
+        from .autotune import (lookup_batch, lookup_one_lane, lookup_prune,
+                               lookup_target_ratio)
         ends = offsets.long()
         sizes = torch.diff(ends, prepend=ends.new_zeros(1)).clamp_min(1)
         # one host read: per-cloud sizes and first-guess candidates per query
@@ -140,6 +144,11 @@ def build_exact_grid(g: Grid, xyz, offsets, k, *, cell=None, scale=None,
         if prune is None:
             g.prune_hint = vote(lookup_prune)
         g.one_lane_hint = vote(lookup_one_lane)
+        weight = {}
+        for nc, kc, dc in zip(ns, ks, dcs):
+            b = lookup_batch(nc, kc, dc)
+            weight[b] = weight.get(b, 0) + nc
+        g.batch_hint = max(weight, key=lambda b: (weight[b], b == "default"))
         if skip_band is not None and all(
                 1.0 / skip_band <= mc / (r * kc) <= skip_band
                 for mc, r, kc in zip(mean, ratios, ks)):
@@ -169,7 +178,8 @@ class ExactKNN:
         self._cap = (n2, B2, q2)
 
     def build(self, xyz, offsets, k, *, cell=None, scale=None, target_ratio=None,
-              adapt=True, nq=0, no_sync=None, prune=None, skip_band=None):
+              adapt=True, nq=0, no_sync=None, prune=None, skip_band=None,
+              min_points=ADAPT_MIN_POINTS):
         """Build the support grid; returns the per-cloud cell side.  The
         FK_QUERY_PRUNE choice for queries on it is ``self.grid.prune_hint``.
 
@@ -190,7 +200,8 @@ class ExactKNN:
             self._reserve(xyz.shape[0], offsets.shape[0], max(nq, xyz.shape[0]))
         self.side = build_exact_grid(self.grid, xyz, offsets, k, cell=cell, scale=scale,
                                      target_ratio=target_ratio, adapt=adapt,
-                                     no_sync=no_sync, prune=prune, skip_band=skip_band)
+                                     no_sync=no_sync, prune=prune, skip_band=skip_band,
+                                     min_points=min_points)
         return self.side
 
     @torch.no_grad()
@@ -235,7 +246,7 @@ class ExactKNN:
                              qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32),
                              no_sync=no_sync)
         out = g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, attached=attached,
-                      prune=g.prune_hint, one_lane=g.one_lane_hint, **qkw)
+                      **g.hints(), **qkw)
         g.status()  # the one host read: reports any device-side error
         return out
 

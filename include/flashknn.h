@@ -155,6 +155,22 @@ typedef struct fk_query_params {
                               to 8 lanes per query, light ones down to 1, so
                               that no wave's scan runs far longer than the
                               rest (results never change)                 */
+#define FK_QUERY_BATCH_SMALL 8 /* candidate batch of the first pass: the
+                                  records each lane loads per scan round
+                                  before the wave checks its LDS key
+                                  buffers.  Default 8 (k <= 16) / 4 (k in
+                                  17..32); SMALL halves it, LARGE takes
+                                  12 / 8 (more loads in flight, fewer
+                                  checks, less buffer room: more merges).
+                                  First passes of k <= 32 on dense grids
+                                  (fk_query; the batched kernels keep the
+                                  default); not both flags.  Compiled variants chosen at
+                                  run time, like the reference's tile
+                                  variants (FlashKNN/csrc/flash_knn_query_
+                                  dynamic_load.cu:903-945); the autotuner
+                                  picks one per bucket.  Results never
+                                  change                                  */
+#define FK_QUERY_BATCH_LARGE 16
 
 typedef struct fk_grid fk_grid;
 

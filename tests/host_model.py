@@ -135,7 +135,7 @@ class Handle:
     def ensure_clouds(self, B):
         self.grow("bbox", box_words(B))
         self.grow("clouds", B)
-        self.grow("prefix", B + 1)
+        self.grow("prefix", 4 * (B + 1))  # one work-item prefix per batched K class
         self.grow("acc", B)
         self.max_clouds = max(self.max_clouds, B)
 
