@@ -56,8 +56,8 @@ def events_ms(fn, reps, stream=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--quick", action="store_true", help="headline only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -295,25 +295,43 @@ def main():
         wts = [torch.randn(s.shape[0], 64, device=dev) for s in stages]
 
         offs = [torch.tensor([s.shape[0]], dtype=torch.int32, device=dev) for s in stages]
+        knns = [fk.StaticKNN() for _ in range(len(stages) - 1)]
 
         def fwd_bwd():
-            loss = 0.0
-            leaves = []
+            # loss = sum_l <out_l, wts_l>: its gradient w.r.t. out_l is wts_l
+            outs, leaves = [], []
             for l in range(len(stages) - 1):
                 q = stages[l].clone().requires_grad_(True)
                 p = stages[l + 1].clone().requires_grad_(True)
                 f = feats[l + 1].clone().requires_grad_(True)
                 leaves += [q, p, f]
-                idx, d = fk.knn(p.detach(), offs[l + 1], 16, q.detach(), offs[l])
-                out = fk.idw_interpolate(f, q, p, idx, eps=1e-8, d=d)
-                loss = loss + (out * wts[l]).sum()
-            loss.backward()
+                idx, _ = knns[l](p.detach(), offs[l + 1], 16, q.detach(), offs[l])
+                outs.append(fk.idw_interpolate(f, q, p, idx, eps=1e-8))
+            torch.autograd.backward(outs, wts[:len(outs)])
             return leaves
 
-        fwd_bwd()
-        h_med, _ = events_ms(fwd_bwd, 10)
+        fwd_bwd()  # freezes the per-level cell sizes
+        h_eager, _ = events_ms(fwd_bwd, 10)
+        # the steady-state step captured once into a graph and replayed
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            fwd_bwd()
+        torch.cuda.current_stream().wait_stream(side)
+        hgraph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(hgraph):
+            cap_leaves = fwd_bwd()
+        ref_leaves = fwd_bwd()
+        hgraph.replay()
+        torch.cuda.synchronize()
+        same_h = all(torch.equal(a.grad, b.grad) for a, b in zip(cap_leaves, ref_leaves))
+        h_med, _ = events_ms(hgraph.replay, 20)
+        for kn in knns:
+            kn.status()
         extra["hierarchy_sizes"] = [int(s.shape[0]) for s in stages]
         extra["hierarchy_fwd_bwd_ms"] = round(h_med, 4)
+        extra["hierarchy_fwd_bwd_eager_ms"] = round(h_eager, 4)
+        extra["hierarchy_graph_grads_equal_eager"] = bool(same_h)
 
     # ------------------------------------------------------- CPU baseline
     cpu = None

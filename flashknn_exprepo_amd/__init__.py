@@ -24,12 +24,12 @@ from .functions import (AdaptiveNeighborhoodFlashKNN, FlashKNN, build_adaptive_o
                         cal_recall, select_adaptive_levels, vanilla_Knn, xyz2key)
 from .grid import Grid
 from .hierarchy import build_flash_hierarchy, voxel_first
-from .knn import ExactKNN, cell_sizes, knn
+from .knn import ExactKNN, StaticKNN, cell_sizes, knn
 
 __all__ = [
     "FlashKNN", "AdaptiveNeighborhoodFlashKNN", "build_adaptive_octree",
     "select_adaptive_levels", "xyz2key", "cal_recall", "vanilla_Knn", "pointops", "knn",
-    "ExactKNN", "cell_sizes", "BatchedKNN", "Grid", "gathered_sqdist",
+    "ExactKNN", "StaticKNN", "cell_sizes", "BatchedKNN", "Grid", "gathered_sqdist",
     "idw_gather", "idw_interpolate", "FlashKNNError", "load", "voxel_first",
     "build_flash_hierarchy",
 ]

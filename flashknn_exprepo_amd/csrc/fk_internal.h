@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "flashknn.h"
 
 namespace fk {
@@ -143,6 +145,40 @@ __device__ __forceinline__ int xcd_remap(int bid, int nb) {
   // tiles, which share candidate cells, share that XCD's L2.
   const int x = bid & 7, l = bid >> 3, per = nb >> 3, rem = nb & 7;
   return x * per + (x < rem ? x : rem) + l;
+}
+
+// A host-side device query (occupancy, CU count) cached per device: such
+// calls cost microseconds of host time, and a query issues a few per call.
+// Tag separates the cached quantities; f(dev, &value) computes one.
+constexpr int kCacheDevices = 64;
+template <class Tag, class F>
+inline hipError_t cached_per_device(F &&f, int *out) {
+  static std::atomic<int> cache[kCacheDevices];  // value + 1; 0 = not yet known
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const bool slot = dev >= 0 && dev < kCacheDevices;
+  if (slot) {
+    const int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) {
+      *out = v - 1;
+      return hipSuccess;
+    }
+  }
+  int r = 0;
+  e = f(dev, &r);
+  if (e != hipSuccess) return e;
+  if (slot) cache[dev].store(r + 1, std::memory_order_relaxed);
+  *out = r;
+  return hipSuccess;
+}
+struct CuCountTag {};
+inline hipError_t device_cus(int *cus) {
+  return cached_per_device<CuCountTag>(
+      [](int dev, int *r) {
+        return hipDeviceGetAttribute(r, hipDeviceAttributeMultiprocessorCount, dev);
+      },
+      cus);
 }
 
 // --------------------------------------------------------------- launches

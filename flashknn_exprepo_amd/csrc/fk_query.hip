@@ -2022,6 +2022,12 @@ void launch_plain(const QueryArgs &a, hipStream_t s) {
 
 }  // namespace
 
+// tags of the cached occupancy queries
+template <int KC>
+struct BatchedTag {};
+template <int KC, int CB>
+struct PlannedTag {};
+
 // ------------------------------------------------- per K class launchers
 // (defined and explicitly instantiated only in the unit of their class: the
 // other units see the declarations and do not compile the class's kernels)
@@ -2037,11 +2043,14 @@ hipError_t FK_LAYOUT(launch_first_pass)(const QueryArgs &a, hipStream_t s) {
 template <int KC>
 hipError_t FK_LAYOUT(launch_batched_class)(const QueryArgs &a, int cls, hipStream_t s) {
   constexpr int WPB = waves_per_block<KC>();
-  int dev = 0, cus = 0, nb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int cus = 0, nb = 0;
+  hipError_t e = device_cus(&cus);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batched<KC>, 64 * WPB, 0);
+    e = cached_per_device<BatchedTag<KC>>(
+        [](int, int *r) {
+          return hipOccupancyMaxActiveBlocksPerMultiprocessor(r, k_batched<KC>, 64 * WPB, 0);
+        },
+        &nb);
   if (e != hipSuccess) return e;
   const unsigned grid = (unsigned)((nb < 1 ? 1 : nb) * cus);
   hipLaunchKernelGGL(k_batched<KC>, dim3(grid), dim3(64 * WPB), 0, s, a, cls);
@@ -2049,13 +2058,21 @@ hipError_t FK_LAYOUT(launch_batched_class)(const QueryArgs &a, int cls, hipStrea
 }
 
 #if !FK_COLUMNS
+template <int KC, int CB>
+hipError_t planned_blocks_cb(int *nb) {
+  constexpr int kThreads = 64 * waves_per_block<KC>();
+  return cached_per_device<PlannedTag<KC, CB>>(
+      [](int, int *r) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(r, k_query_planned<KC, CB>,
+                                                            kThreads, 0);
+      },
+      nb);
+}
+
 template <int KC>
 hipError_t planned_blocks(int sel, int *nb) {
   hipError_t e = hipSuccess;
-  with_batch<KC>(sel, [&](auto cb) {
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-        nb, k_query_planned<KC, decltype(cb)::value>, 64 * waves_per_block<KC>(), 0);
-  });
+  with_batch<KC>(sel, [&](auto cb) { e = planned_blocks_cb<KC, decltype(cb)::value>(nb); });
   return e;
 }
 #define FK_PLANNED_BLOCKS(KC) template hipError_t planned_blocks<KC>(int, int *);
@@ -2141,9 +2158,8 @@ hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
 constexpr int kPlanOver = FK_PLAN_OVER;
 
 hipError_t planned_capacity(int kc, int batch_sel, int32_t *waves) {
-  int dev = 0, cus = 0, nb = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  int cus = 0, nb = 0;
+  hipError_t e = device_cus(&cus);
   if (e != hipSuccess) return e;
   int wpb = 0;
   switch (kc) {

@@ -43,6 +43,7 @@ class Grid:
         F.check(self._L.fk_grid_create(ctypes.byref(h)), "fk_grid_create")
         self._h = h
         self._keep = []          # tensors captured by a graph stay alive
+        self._params = {}        # query parameter blocks by their arguments
         self.from_xyz = False
         self.num_points = 0
         self.num_clouds = 0
@@ -223,12 +224,19 @@ class Grid:
             out_idx = torch.empty((nq, k), dtype=idx_dtype, device=dev)
         if dist_mode != F.FK_DIST_NONE and out_dist is None:
             out_dist = torch.empty((nq, k), dtype=torch.float32, device=dev)
-        p = F.make_params(k, mode, radius_cells, cut_r2, out_idx.dtype == torch.int64,
-                          pad_mode, dist_mode, pad_dist,
-                          F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
-                          passes, (F.FK_QUERY_PRUNE if prune else 0)
-                          | (F.FK_QUERY_ONE_LANE if one_lane else 0)
-                          | (F.FK_QUERY_UNIFORM if uniform else 0) | BATCH_FLAGS[batch])
+        key = (k, mode, radius_cells, cut_r2, out_idx.dtype == torch.int64, pad_mode,
+               dist_mode, pad_dist, attached, passes, prune, one_lane, uniform, batch)
+        p = self._params.get(key)
+        if p is None:  # the parameter block of a repeated query is built once
+            p = F.make_params(k, mode, radius_cells, cut_r2, out_idx.dtype == torch.int64,
+                              pad_mode, dist_mode, pad_dist,
+                              F.FK_QUERIES_ATTACHED if attached else F.FK_QUERIES_SELF,
+                              passes, (F.FK_QUERY_PRUNE if prune else 0)
+                              | (F.FK_QUERY_ONE_LANE if one_lane else 0)
+                              | (F.FK_QUERY_UNIFORM if uniform else 0) | BATCH_FLAGS[batch])
+            if len(self._params) > 64:
+                self._params.clear()
+            self._params[key] = p
         F.check(self._L.fk_query(self._h, ctypes.byref(p), F.ptr(out_idx),
                                  F.ptr(out_dist), F.stream_ptr()), "fk_query")
         return out_idx, out_dist

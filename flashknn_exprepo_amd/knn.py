@@ -251,6 +251,51 @@ class ExactKNN:
         return out
 
 
+class StaticKNN:
+    """Exact kNN for a step that repeats on inputs of fixed shape (a
+    training step's level-to-level queries), capturable into a graph.
+
+    The first call runs ``ExactKNN`` eagerly (density feedback, host reads)
+    and freezes the per-cloud cell sizes and speed hints it chose.  Later
+    calls build with those sizes into the reserved capacity and query
+    attached points with the counter reset included (no host read, no new
+    device table once the first static call has sized them), so a
+    ``torch.cuda.graph`` can capture them; outputs are the same exact kNN.
+    Errors of captured calls (a box that outgrew the reserve) surface at
+    ``status()``."""
+
+    def __init__(self):
+        self.eng = ExactKNN()
+        self.inv = None
+
+    def status(self):
+        self.eng.grid.status()
+
+    @torch.no_grad()
+    def __call__(self, xyz, offsets, k, new_xyz=None, new_offsets=None, *,
+                 idx_dtype=torch.int64, pad_mode=F.FK_PAD_MINUS1):
+        if self.inv is None:
+            out = self.eng(xyz, offsets, k, new_xyz, new_offsets, idx_dtype=idx_dtype,
+                           pad_mode=pad_mode)
+            self.inv = (1.0 / self.eng.side.double()).float()
+            self.hints = self.eng.grid.hints()
+            return out
+        F.require_device(xyz, "xyz")
+        xyz = xyz.float().contiguous()
+        offsets = offsets.to(device=xyz.device, dtype=torch.int32).contiguous()
+        attached = new_xyz is not None
+        nq = new_xyz.shape[0] if attached else 0
+        self.eng._reserve(xyz.shape[0], offsets.shape[0], max(nq, xyz.shape[0]))
+        g = self.eng.grid
+        g.build_xyz(xyz, offsets, self.inv, no_sync=True)
+        if attached:
+            g.attach_queries(new_xyz.float().contiguous(),
+                             qoffsets=new_offsets.to(device=xyz.device, dtype=torch.int32),
+                             no_sync=True)
+        return g.query(k, mode=F.FK_MODE_EXACT, radius_cells=1, attached=attached,
+                       idx_dtype=idx_dtype, pad_mode=pad_mode, passes=7, **self.hints)
+
+
 _engines = {}
 
 
