@@ -55,18 +55,24 @@ def _load():
 
 
 def _entry(n: int, k: int, dclass: int):
-    """Table entry of the bucket; else the nearest tuned size with the same
-    k class and density class; else None."""
+    """Table entry of the bucket; else the nearest tuned bucket of the same
+    k class (distance |log2 n step| + |density class step|, the same density
+    class first among equals); else None."""
     with _LOCK:
         t = _load()
     e = t.get(bucket(n, k, dclass))
     if e:
         return e
-    suffix = f"_k{_kclass(k)}_d{int(dclass)}"
+    kc = f"_k{_kclass(k)}_d"
     ln = max(0, int(math.log2(max(n, 1))))
-    near = [(abs(int(key[1:key.index("_")]) - ln), v) for key, v in t.items()
-            if key.endswith(suffix)]
-    return min(near, key=lambda x: x[0])[1] if near else None
+    near = []
+    for key, v in t.items():
+        if kc not in key:
+            continue
+        kn = int(key[1:key.index("_")])
+        kd = int(key[key.rindex("_d") + 2:])
+        near.append((abs(kn - ln) + abs(kd - int(dclass)), kd != int(dclass), abs(kn - ln), v))
+    return min(near, key=lambda x: x[:3])[3] if near else None
 
 
 def lookup_target_ratio(n: int, k: int, dclass: int = 0) -> float:

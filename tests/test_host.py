@@ -104,6 +104,14 @@ def test_autotune_lookup_and_density_class(tmp_path, monkeypatch):
     monkeypatch.setattr(autotune, "_TABLE", None)
     assert autotune.lookup_prune(90000, 16, 5) is True
     assert autotune.lookup_target_ratio(90000, 16, 5) == 19.0
+    # a far size of the same density class loses to a near size one class off
+    table = {autotune.bucket(1500, 16, 5): {"target_ratio": 28.0},
+             autotune.bucket(90000, 16, 4): {"target_ratio": 9.0, "prune": True}}
+    p.write_text(json.dumps(table))
+    monkeypatch.setattr(autotune, "_TABLE", None)
+    assert autotune.lookup_target_ratio(250000, 16, 5) == 9.0
+    assert autotune.lookup_prune(250000, 16, 5) is True
+    assert autotune.lookup_target_ratio(2000, 16, 5) == 28.0
 
 
 def test_quotient32_correctly_rounded():
