@@ -45,6 +45,7 @@ FK_QUERY_UNIFORM = 4
 FK_QUERY_BATCH_SMALL = 8
 FK_QUERY_BATCH_LARGE = 16
 FK_VOXEL_NOALLOC_MAX_POINTS = 1048576
+FK_ADAPTIVE_STATS = 72
 
 
 class FlashKNNError(RuntimeError):
@@ -126,6 +127,10 @@ SIGNATURES = [
                                              ctypes.c_float, _vp, _vp, _vp]),
     ("fk_idw_interp_backward", ctypes.c_int, [_vp, _i64, _vp, _i64, _vp, _i32, _vp, _i32,
                                               ctypes.c_float, _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("fk_adaptive_workspace_bytes", ctypes.c_int64, [_i64]),
+    ("fk_adaptive_select", ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32, _vp,
+                                          _vp, _vp, _vp]),
+    ("fk_adaptive_query", ctypes.c_int, [_vp, _i64, _i32, ctypes.c_float, _vp, _vp, _vp, _vp]),
     ("fk_voxel_workspace_bytes", ctypes.c_int64, [_i64]),
     ("fk_voxel_first", ctypes.c_int, [_vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp, _vp, _vp]),
     ("fk_voxel_first_noalloc", ctypes.c_int, [_vp, _vp, _i32, _i64, ctypes.c_float, _vp, _vp, _vp]),

@@ -1044,6 +1044,53 @@ int64_t fk_voxel_workspace_bytes(int64_t n) {
   return voxel_workspace_bytes(n);
 }
 
+int64_t fk_adaptive_workspace_bytes(int64_t n) {
+  if (n < 1 || n > INT32_MAX / 2) return -1;
+  return adaptive_workspace_bytes(n);
+}
+
+static_assert(FK_ADAPTIVE_STATS == kAdStats, "adaptive stats block");
+
+int fk_adaptive_select(const int64_t *grid_coord, const int64_t *batch_idx,
+                       const float *xyz, int64_t n, int32_t k, int32_t min_factor,
+                       int32_t max_factor, int32_t max_shift, void *workspace,
+                       int32_t *out_level, int64_t *stats, void *stream) {
+  if (k < 1 || k > kMaxLaneK)
+    return fail(FK_ERR_K_RANGE, "fk_adaptive_select: k = %d outside [1, %d]", k, kMaxLaneK);
+  if (n < 1 || n > INT32_MAX / 2 || !grid_coord || !workspace || !stats || min_factor < 1 ||
+      max_factor < min_factor || (int64_t)max_factor * k > INT32_MAX || max_shift < -1 ||
+      max_shift > 16)
+    return fail(FK_ERR_INVALID_ARG, "fk_adaptive_select: bad arguments");
+  FK_HIP(launch_adaptive_select(grid_coord, batch_idx, xyz, n, min_factor * k,
+                                max_factor * k, max_shift, workspace, out_level, stats,
+                                S(stream)));
+  return ok();
+}
+
+int fk_adaptive_query(void *workspace, int64_t n, int32_t k, float cut_r2,
+                      const int64_t *stats, int64_t *out_idx, float *out_dist,
+                      void *stream) {
+  if (k < 1 || k > kMaxLaneK)
+    return fail(FK_ERR_K_RANGE, "fk_adaptive_query: k = %d outside [1, %d]", k, kMaxLaneK);
+  if (n < 1 || n > INT32_MAX / 2 || !workspace || !stats || !out_idx || std::isnan(cut_r2))
+    return fail(FK_ERR_INVALID_ARG, "fk_adaptive_query: bad arguments");
+  AdaptiveArgs a;
+  memset(&a, 0, sizeof(a));
+  FK_HIP(adaptive_tables(workspace, n, stats, &a));
+  a.k = k;
+  a.cut_r2 = cut_r2 > 0.f ? cut_r2 : 0.f;
+  a.out_idx = out_idx;
+  a.out_d = out_dist;
+  hipStream_t s = S(stream);
+  switch (kclass_of(k)) {
+    case 8: FK_HIP(launch_adaptive_query<8>(a, s)); break;
+    case 16: FK_HIP(launch_adaptive_query<16>(a, s)); break;
+    case 32: FK_HIP(launch_adaptive_query<32>(a, s)); break;
+    default: FK_HIP(launch_adaptive_query<64>(a, s)); break;
+  }
+  return ok();
+}
+
 int fk_voxel_first(const float *xyz, const int32_t *offsets, int32_t num_clouds,
                    int64_t n, float voxel_size, void *workspace,
                    int64_t *out_idx, int64_t *out_count, void *stream) {

@@ -42,17 +42,7 @@ __global__ void k_fill_dev(int32_t *p, int32_t v, const int32_t *n_dev,
 
 // ------------------------------------------------------------------ Morton
 
-// Bit spreading by the standard magic-number interleave: bit i of v moves to
-// bit 3i.
-__device__ __forceinline__ uint64_t spread3(uint64_t v) {
-  v &= 0xffffull;
-  v = (v | (v << 16)) & 0x0000ff0000ffull;
-  v = (v | (v << 8)) & 0x00f00f00f00full;
-  v = (v | (v << 4)) & 0x0c30c30c30c3ull;
-  v = (v | (v << 2)) & 0x249249249249ull;
-  return v;
-}
-
+// (spread3: fk_internal.h)
 __device__ __forceinline__ int64_t morton(int64_t x, int64_t y, int64_t z,
                                           int64_t b) {
   return (int64_t)((spread3((uint64_t)x) << 2) | (spread3((uint64_t)y) << 1) |

@@ -181,6 +181,167 @@ inline hipError_t device_cus(int *cus) {
       cus);
 }
 
+// ------------------------------------------------ Morton keys (16 bits/axis)
+// Bit i of v moves to bit 3i (the standard magic-number interleave), and
+// back.  A key holds x at bit 3i+2, y at 3i+1, z at 3i and the batch above
+// bit 48 (the reference's xyz2key layout, FlashKNN/functions/z_order.py).
+__host__ __device__ __forceinline__ uint64_t spread3(uint64_t v) {
+  v &= 0xffffull;
+  v = (v | (v << 16)) & 0x0000ff0000ffull;
+  v = (v | (v << 8)) & 0x00f00f00f00full;
+  v = (v | (v << 4)) & 0x0c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x249249249249ull;
+  return v;
+}
+__host__ __device__ __forceinline__ uint32_t compact3(uint64_t v) {
+  v &= 0x249249249249ull;
+  v = (v | (v >> 2)) & 0x0c30c30c30c3ull;
+  v = (v | (v >> 4)) & 0x00f00f00f00full;
+  v = (v | (v >> 8)) & 0x0000ff0000ffull;
+  v = (v | (v >> 16)) & 0xffffull;
+  return (uint32_t)v;
+}
+__host__ __device__ __forceinline__ uint64_t morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return (spread3(x) << 2) | (spread3(y) << 1) | spread3(z);
+}
+
+// ------------------------------------------- adaptive neighbourhoods
+// (fk_adaptive.hip builds, the query kernel is in fk_query.hip)
+//
+// Points sorted by key = batch << 48 | Morton(shifted coordinates): the
+// cell of a point at level s (coordinates >> s) is key >> 3s, and the points
+// of one cell of any level are one run of the sorted order.  Each level
+// keeps an open-addressing table of 16-byte entries, cell key -> (run start,
+// run length), ad_slots(n) entries per level (load <= 1/2), plus the cell's
+// 27-cell candidate count in a parallel array.
+constexpr int kAdLevels = 17;        // shifts 0..16
+constexpr int kAdMaxBatch = 32768;   // batch indices in [0, 2^15)
+constexpr uint64_t kAdEmpty = ~0ull;
+constexpr int kAdStats = 4 + 4 * kAdLevels;
+// stats (int64, device): status, reason, largest shifted coordinate, the
+// coarsest level used, then per level s: points that selected s, and the
+// sum / min / max of their 27-cell counts
+enum AdStat : int { kAdStatus = 0, kAdReason = 1, kAdGmax = 2, kAdSmax = 3, kAdLevel0 = 4 };
+constexpr int kAdReasonBatch = 1;    // a batch index outside [0, 2^15)
+constexpr int kAdReasonSpan = 2;     // a batch spans 2^16 cells or more on an axis
+
+struct alignas(16) AdEntry {
+  uint64_t key;    // cell key at the entry's level, kAdEmpty when free
+  int32_t start;   // first sorted position of the cell's run
+  int32_t count;   // run length
+};
+
+__host__ __device__ __forceinline__ int ad_log2_slots(int64_t n) {
+  int lg = 6;
+  while ((int64_t(1) << lg) < 2 * n) ++lg;
+  return lg;
+}
+__host__ __device__ __forceinline__ int64_t ad_slots(int64_t n) {
+  return int64_t(1) << ad_log2_slots(n);
+}
+// Home slot of cell key c in a table of 2^lg entries: Fibonacci hashing,
+// the top bits of c * 2^64 / phi.  (Keeping the 8 cells of a 2x2x2 block in
+// one 128-byte line made absent neighbours probe through full lines: the
+// 27-cell counts of a 92k-point sweep took 214 instead of 127 us.)
+__host__ __device__ __forceinline__ int64_t ad_home(uint64_t c, int lg) {
+  return (int64_t)(((c ^ (c >> 31)) * 0x9e3779b97f4a7c15ull) >> (64 - lg));
+}
+// Entry index of cell c at level s, or -1 (the table always has a free slot).
+__device__ __forceinline__ int64_t ad_find(const AdEntry *__restrict__ ent, int lg, int s,
+                                           uint64_t c) {
+  const int64_t P = int64_t(1) << lg;
+  const AdEntry *t = ent + (int64_t)s * P;
+  int64_t h = ad_home(c, lg);
+  for (;;) {
+    const uint64_t k = t[h].key;
+    if (k == c) return (int64_t)s * P + h;
+    if (k == kAdEmpty) return -1;
+    h = (h + 1) & (P - 1);
+  }
+}
+// N cells of level s at once: the first probes are independent 16-byte
+// loads, all in flight together (the rare longer chains follow); entries of
+// absent cells (and of c = kAdEmpty) come back as {kAdEmpty, 0, 0}.
+template <int N>
+__device__ __forceinline__ void ad_find_n(const AdEntry *__restrict__ ent, int lg, int s,
+                                          const uint64_t (&c)[N], AdEntry (&out)[N],
+                                          int64_t (&slot)[N]) {
+  const int64_t P = int64_t(1) << lg;
+  const AdEntry *t = ent + (int64_t)s * P;
+  int64_t h[N];
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    h[j] = ad_home(c[j], lg);
+    out[j] = c[j] == kAdEmpty ? AdEntry{kAdEmpty, 0, 0} : t[h[j]];
+  }
+#pragma unroll
+  for (int j = 0; j < N; ++j) {
+    while (out[j].key != c[j] && out[j].key != kAdEmpty) {
+      h[j] = (h[j] + 1) & (P - 1);
+      out[j] = t[h[j]];
+    }
+    const bool hit = c[j] != kAdEmpty && out[j].key == c[j];
+    slot[j] = hit ? (int64_t)s * P + h[j] : -1;
+    if (!hit) out[j] = AdEntry{kAdEmpty, 0, 0};
+  }
+}
+
+// Cells of level s around cell c: decode once, spread each axis's three
+// coordinates once, and combine (27 keys from 9 spreads); kAdEmpty where a
+// coordinate leaves [0, 2^(16-s)).  Neighbour j is (dx, dy, dz) =
+// (j / 9 - 1, j / 3 % 3 - 1, j % 3 - 1).
+struct AdNeighbours {
+  uint64_t hi;            // batch bits of the level's key
+  uint64_t sp[3][3];      // spread coordinates per axis and offset
+  bool ok[3][3];
+  __device__ __forceinline__ void init(uint64_t c, int s) {
+    const int mb = 48 - 3 * s;
+    const uint64_t m = mb > 0 ? (c & ((1ull << mb) - 1)) : 0ull;
+    hi = c - m;
+    const int lim = 1 << (16 - s);
+    const int v[3] = {(int)compact3(m >> 2), (int)compact3(m >> 1), (int)compact3(m)};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int x = v[a] + d - 1;
+        ok[a][d] = x >= 0 && x < lim;
+        sp[a][d] = spread3((uint64_t)(ok[a][d] ? x : 0)) << (2 - a);
+      }
+  }
+  __device__ __forceinline__ uint64_t key(int j) const {
+    const int dx = j / 9, dy = (j / 3) % 3, dz = j % 3;
+    return (ok[0][dx] && ok[1][dy] && ok[2][dz]) ? (hi | sp[0][dx] | sp[1][dy] | sp[2][dz])
+                                                 : kAdEmpty;
+  }
+};
+
+struct AdaptiveArgs {
+  const uint64_t *skeys;   // [n] sorted keys
+  const float4 *srec;      // [n] sorted (x, y, z, input index bits)
+  const int32_t *level;    // [n] selected level of each sorted point
+  const AdEntry *ent;      // [kAdLevels * P]
+  const int32_t *hc27;     // [kAdLevels * P]
+  const int64_t *stats;    // [kAdStats]
+  int32_t lg;              // P = 2^lg entries per level
+  int32_t n;
+  int32_t k;
+  float cut_r2;
+  int64_t *out_idx;        // [n, k] rows in input order
+  float *out_d;            // [n, k] squared distances (nullable)
+};
+// The 27-cell query at each point's selected level, K class KC (fk_query.hip).
+template <int KC>
+hipError_t launch_adaptive_query(const AdaptiveArgs &a, hipStream_t s);
+// Sort, tables, counts and level selection (fk_adaptive.hip); the tables of
+// a workspace it filled, for the query.
+int64_t adaptive_workspace_bytes(int64_t n);
+hipError_t launch_adaptive_select(const int64_t *grid, const int64_t *batch,
+                                  const float *xyz, int64_t n, int32_t lo, int32_t hi,
+                                  int32_t max_shift, void *ws, int32_t *out_level,
+                                  int64_t *stats, hipStream_t s);
+hipError_t adaptive_tables(void *ws, int64_t n, const int64_t *stats, AdaptiveArgs *args);
+
 // --------------------------------------------------------------- launches
 // (defined in fk_build.hip / fk_query.hip / fk_backward.hip)
 

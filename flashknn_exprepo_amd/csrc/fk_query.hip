@@ -690,7 +690,9 @@ struct ColumnStream {
   }
 };
 
-template <int KC, int G, int NC, int S>
+// GT: the lanes that share one query's threshold (G, unless the group's
+// lanes scan different ranges whole).
+template <int KC, int G, int NC, int S, int GT = G>
 __device__ __forceinline__ void scan_stream(TopK<KC> &t, ColumnStream<NC, G, S> &cs,
                                             float4 qr,
                                             const float4 *__restrict__ srec) {
@@ -713,7 +715,7 @@ __device__ __forceinline__ void scan_stream(TopK<KC> &t, ColumnStream<NC, G, S> 
     }
     if (!wave_any(cs.lc > 0)) break;
     t.flush();
-    t.template share_threshold<G>();
+    t.template share_threshold<GT>();
   }
 }
 
@@ -1093,6 +1095,91 @@ k_query_planned(QueryArgs a) {
     default: qx = run_tile<KC, 8, true, CB>(a, q0, qend, wb + lane, wt + lane); break;
   }
   if (qx >= 0) inline_expand<KC>(a, qx, lane, wb, wt);
+}
+
+// ------------------------------------------- adaptive neighbourhood query
+//
+// Point p of the Morton-sorted order scans the 27 cells around its cell at
+// its selected level (FlashKNN/functions/adaptive_octree.py:369-488 flattens
+// the same query into one launch).  A group of G lanes serves one point and
+// one wave 64 / G consecutive points, which mostly share cells and levels.
+// Lane `sub` of the group looks up the cells j = sub, sub + G, ... of the 27
+// in its level's table (the lookups in flight together); the runs then reach
+// every lane of the group by shuffles, nine cells at a time, and lane `sub`
+// takes every G-th record of them (the group's loads adjacent, its lanes'
+// work even).  (Two lanes each looking up all 27 cells: 160 us for 92k
+// points; four lanes each streaming whole runs of its own cells: 214 us.)
+// Rows go to the input order; slots without a candidate hold (the point
+// itself, +inf).
+#ifndef FK_AD_G
+#define FK_AD_G 4
+#endif
+constexpr int kAdG = FK_AD_G;
+constexpr int kAdCells = (27 + kAdG - 1) / kAdG;  // cells looked up per lane
+
+template <int KC>
+__global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<KC>())
+k_adaptive_query(AdaptiveArgs a) {
+  constexpr int WPB = waves_per_block<KC>();
+  constexpr int BS = buffer_slots<KC>();
+  constexpr int G = kAdG;
+  constexpr int NC = kAdCells;
+  constexpr int S = kSlots <= BS ? kSlots : BS;
+  static_assert(G >= 2 && G <= 16 && (G & (G - 1)) == 0, "adaptive query: lanes per point");
+  __shared__ uint64_t sbuf[WPB * BS * kWave];
+  __shared__ uint64_t stab[WPB * kStreamCols * kWave];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
+  const int sub = lane & (G - 1), gbase = lane & ~(G - 1);
+  const int q0 = (xcd_remap(blockIdx.x, gridDim.x) * WPB + w) * (kWave / G);
+  // wave-uniform exits: past the end, or the selection raised an error
+  if (q0 >= a.n || a.stats[kAdStatus] != 0) return;
+  const int qi = q0 + lane / G;
+  const bool active = qi < a.n;
+  const int qs = active ? qi : a.n - 1;
+  const float4 qr = a.srec[qs];
+  const int s = a.level[qs];
+  const uint64_t c = a.skeys[qs] >> (3 * s);
+  AdNeighbours nb;
+  nb.init(c, s);
+  uint64_t cells[NC];
+#pragma unroll
+  for (int i = 0; i < NC; ++i) cells[i] = kAdEmpty;
+#pragma unroll
+  for (int j = 0; j < 27; ++j)
+    if (active && (j & (G - 1)) == sub) cells[j / G] = nb.key(j);
+  AdEntry own[NC];
+  int64_t slot[NC];
+  ad_find_n<NC>(a.ent, a.lg, s, cells, own, slot);
+  uint64_t *lane_buf = sbuf + w * BS * kWave + lane;
+  uint64_t *lane_tab = stab + w * kStreamCols * kWave + lane;
+  TopK<KC> t;
+  t.init(lane_buf, a.k, a.cut_r2, __float_as_uint(qr.w));
+#pragma unroll
+  for (int part = 0; part < 3; ++part) {
+    int b[9], e[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) {
+      const int j = 9 * part + m;
+      const int from = gbase + (j & (G - 1));
+      b[m] = __shfl(own[j / G].start, from, kWave);
+      e[m] = b[m] + __shfl(own[j / G].count, from, kWave);
+    }
+    ColumnStream<9, G, S> cs;
+    cs.init(lane_tab, b, e, sub);
+    scan_stream<KC, G, 9, S>(t, cs, qr, a.srec);
+  }
+  if (wave_any(t.cnt > 0)) t.flush();
+  t.template group_merge<G>(lane);
+  if (!active) return;
+  const int64_t ob = (int64_t)__float_as_int(qr.w) * a.k - t.base();
+#pragma unroll
+  for (int j = 0; j < KC; ++j) {
+    if (j >= t.base() && ((j - t.base()) & (G - 1)) == sub) {
+      const uint64_t key = t.top[j];
+      a.out_idx[ob + j] = (int64_t)key_id(key);
+      if (a.out_d) a.out_d[ob + j] = __uint_as_float(key_dbits(key));
+    }
+  }
 }
 
 // ------------------------------------------------- cooperative expansion
@@ -2080,12 +2167,30 @@ hipError_t planned_blocks(int sel, int *nb) {
 #define FK_PLANNED_BLOCKS(KC)
 #endif
 
+#if !FK_COLUMNS
+template <int KC>
+hipError_t launch_adaptive_query(const AdaptiveArgs &a, hipStream_t s) {
+  constexpr int WPB = waves_per_block<KC>();
+  constexpr int QPW = kWave / kAdG;
+  if (a.n <= 0) return hipSuccess;
+  const int64_t waves = ((int64_t)a.n + QPW - 1) / QPW;
+  const unsigned nb = (unsigned)((waves + WPB - 1) / WPB);
+  hipLaunchKernelGGL(k_adaptive_query<KC>, dim3(nb), dim3(64 * WPB), 0, s, a);
+  return hipGetLastError();
+}
+#define FK_ADAPTIVE_UNIT(KC) \
+  template hipError_t launch_adaptive_query<KC>(const AdaptiveArgs &, hipStream_t);
+#else
+#define FK_ADAPTIVE_UNIT(KC)
+#endif
+
 #define FK_CLASS_UNIT(KC)                                                        \
   template hipError_t FK_LAYOUT(launch_first_pass)<KC>(const QueryArgs &,        \
                                                        hipStream_t);             \
   template hipError_t FK_LAYOUT(launch_batched_class)<KC>(const QueryArgs &, int, \
                                                           hipStream_t);          \
-  FK_PLANNED_BLOCKS(KC)
+  FK_PLANNED_BLOCKS(KC)                                                          \
+  FK_ADAPTIVE_UNIT(KC)
 #if FK_CLASS(8)
 FK_CLASS_UNIT(8)
 #endif
@@ -2098,6 +2203,7 @@ FK_CLASS_UNIT(32)
 #if FK_CLASS(64)
 template hipError_t FK_LAYOUT(launch_first_pass)<64>(const QueryArgs &, hipStream_t);
 template hipError_t FK_LAYOUT(launch_batched_class)<64>(const QueryArgs &, int, hipStream_t);
+FK_ADAPTIVE_UNIT(64)
 #endif
 #endif  // class units
 

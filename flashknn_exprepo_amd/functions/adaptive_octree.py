@@ -209,7 +209,9 @@ class AdaptiveNeighborhoodFlashKNN:
         self.min_candidates_factor = min_candidates_factor
         self.max_candidates_factor = max_candidates_factor
         self.last_stats: dict[str, Any] = {}
+        self.last_levels = None      # selected level per point (int32), k <= 64
         self._grids: dict[int, Grid] = {}
+        self._ws = None              # workspace of the k <= 64 path
 
     def _level_grid(self, s: int) -> Grid:
         """Grid of level s, kept between calls (its device tables are reused)."""
@@ -223,6 +225,102 @@ class AdaptiveNeighborhoodFlashKNN:
               xyz: torch.Tensor | None = None, *, max_shift: int | None = None,
               batch_for_prune: int = 1, cut_radius: float = math.inf,
               return_distances: bool = False):
+        """k <= 64: two library calls (fk_adaptive_select, fk_adaptive_query)
+        over one Morton sort and per-level cell tables, and one host read of
+        the statistics at the end.  Larger k: one grid per level (_query_levels)."""
+        K = self.num_nbr
+        if K < 1 or K > F.FK_MAX_K:
+            raise ValueError(f"num_nbr must be in [1, {F.FK_MAX_K}]")
+        if K <= F.FK_MAX_BATCHED_K:
+            return self._query_device(grid_coord, batch_idx, xyz, max_shift, cut_radius,
+                                      return_distances)
+        return self._query_levels(grid_coord, batch_idx, xyz, max_shift=max_shift,
+                                  cut_radius=cut_radius, return_distances=return_distances)
+
+    def _workspace(self, n, dev):
+        nbytes = int(F.load().fk_adaptive_workspace_bytes(n))
+        if nbytes < 0:
+            raise ValueError(f"adaptive query: {n} points is above the supported size")
+        ws = self._ws
+        if ws is None or ws.numel() < nbytes or ws.device != dev:
+            ws = self._ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+        return ws
+
+    def _query_device(self, grid_coord, batch_idx, xyz, max_shift, cut_radius,
+                      return_distances):
+        K = self.num_nbr
+        if not grid_coord.is_cuda or not batch_idx.is_cuda:
+            raise ValueError("adaptive octree inputs must be HIP device tensors")
+        if grid_coord.dtype != torch.int64 or batch_idx.dtype != torch.int64:
+            raise ValueError("grid_coord and batch_idx must use torch.int64")
+        if grid_coord.ndim != 2 or grid_coord.shape[1] != 3:
+            raise ValueError("grid_coord must have shape [N, 3]")
+        if batch_idx.ndim != 1 or batch_idx.shape[0] != grid_coord.shape[0]:
+            raise ValueError("batch_idx must have shape [N]")
+        n = grid_coord.shape[0]
+        if n == 0:
+            raise ValueError("cannot construct an octree for an empty point cloud")
+        if max_shift is not None and (max_shift < 0 or max_shift > 16):
+            raise ValueError("max_shift must lie in [0, 16]")
+        if n < K:
+            raise ValueError(f"point cloud contains {n} points, fewer than k={K}")
+        if self.min_candidates_factor < 1:
+            raise ValueError("min_candidates_factor must be at least one")
+        if self.max_candidates_factor < self.min_candidates_factor:
+            raise ValueError("max_candidates_factor must not be smaller than min")
+        dev = grid_coord.device
+        L = F.load()
+        g = grid_coord.contiguous()
+        b = batch_idx.contiguous()
+        pts = None if xyz is None else xyz.float().contiguous()
+        ws = self._workspace(n, dev)
+        stats = torch.empty(F.FK_ADAPTIVE_STATS, dtype=torch.int64, device=dev)
+        level = torch.empty(n, dtype=torch.int32, device=dev)
+        out_i = torch.empty((n, K), dtype=torch.int64, device=dev)
+        out_d = torch.empty((n, K), dtype=torch.float32, device=dev) if return_distances else None
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        s = F.stream_ptr()
+        ev[0].record()
+        F.check(L.fk_adaptive_select(F.ptr(g), F.ptr(b), F.ptr(pts), n, K,
+                                     self.min_candidates_factor, self.max_candidates_factor,
+                                     -1 if max_shift is None else int(max_shift), F.ptr(ws),
+                                     F.ptr(level), F.ptr(stats), s), "fk_adaptive_select")
+        ev[1].record()
+        F.check(L.fk_adaptive_query(F.ptr(ws), n, K, float(cut_radius) ** 2, F.ptr(stats),
+                                    F.ptr(out_i), F.ptr(out_d), s), "fk_adaptive_query")
+        ev[2].record()
+        st = stats.cpu().tolist()  # the one host read
+        if st[0]:
+            if st[1] == 1:
+                raise ValueError("batch indices must lie in [0, 32768)")
+            if st[1] == 2:
+                raise ValueError("each batch's coordinate span must be smaller than 65536")
+            F.check(int(st[0]), "fk_adaptive_select (device)")
+        smax = int(st[3])
+        t_build, t_query = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+        levels = []
+        for lv in range(smax + 1):
+            m, tot, lo, hi = st[4 + 4 * lv: 8 + 4 * lv]
+            if m:
+                levels.append({"shift": lv, "query_points": int(m), "candidate_mean": tot / m,
+                               "candidate_min": int(lo), "candidate_max": int(hi)})
+        self.last_levels = level
+        self.last_stats = {
+            "construction_ms": t_build, "selection_ms": 0.0, "query_ms": t_query,
+            "total_ms": t_build + t_query, "k": K,
+            "min_candidates_factor": self.min_candidates_factor,
+            "max_candidates_factor": self.max_candidates_factor,
+            "octree_levels": smax + 1, "levels_built": smax + 1,
+            "selected_levels": levels, "query_kernel_launches": 1,
+        }
+        if return_distances:
+            return out_i, out_d
+        return out_i
+
+    @torch.no_grad()
+    def _query_levels(self, grid_coord: torch.Tensor, batch_idx: torch.Tensor,
+                      xyz: torch.Tensor | None = None, *, max_shift: int | None = None,
+                      cut_radius: float = math.inf, return_distances: bool = False):
         """Levels are built from the coarsest down while the selection rule
         runs on the device; the descent stops at the first level where no
         point may go finer (finer levels could not be selected), so the
@@ -231,8 +329,6 @@ class AdaptiveNeighborhoodFlashKNN:
         descend), one for the per-level query counts and one for the
         statistics; no per-point work runs on the host."""
         K = self.num_nbr
-        if K < 1 or K > F.FK_MAX_K:
-            raise ValueError(f"num_nbr must be in [1, {F.FK_MAX_K}]")
         dev = grid_coord.device
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()

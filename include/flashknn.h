@@ -41,6 +41,10 @@
  *   fk_batched_*              no reference counterpart (one persistent launch
  *                             over many clouds with per-cloud K, hipGraph
  *                             replay)
+ *   fk_adaptive_select/query  AdaptiveNeighborhoodFlashKNN.query
+ *                             (FlashKNN/functions/adaptive_octree.py:491-612:
+ *                             build_adaptive_octree, select_adaptive_levels,
+ *                             the flattened query :369-488)
  *   fk_dist_backward          no reference counterpart (FlashKNN.query is
  *                             @torch.no_grad, FlashKnnWrapper.py:165)
  *   fk_idw_forward/backward   pointops interpolation_forward/backward_cuda
@@ -357,6 +361,47 @@ int fk_idw_interp_backward(const float *q, int64_t nq, const float *p, int64_t n
                            int32_t channels, float eps, const float *grad_out,
                            float *grad_feat, float *grad_q, float *grad_p,
                            void *workspace, void *stream);
+
+/* ------------------------------------------- adaptive neighbourhoods */
+/* AdaptiveNeighborhoodFlashKNN.query (FlashKNN/functions/adaptive_octree.py
+ * :491-612) for k <= 64, in two calls with no host synchronisation:
+ *
+ * fk_adaptive_select: shifts each batch's int64 cell coordinates to start
+ * at 0 (:129-146; spans must stay below 2^16, batch indices in [0, 2^15)),
+ * takes the coarsest level max_shift (-1: the bit length of the largest
+ * shifted coordinate) and, per point, the reference's coarse-to-fine rule
+ * over the 27-node candidate counts of levels max_shift .. 0 (:312-351:
+ * descend while the count exceeds max_factor * k, accept a finer level
+ * only while its count is at least min_factor * k).  grid_coord int64[n,3],
+ * batch_idx int64[n] or NULL (one batch), xyz float[n,3] or NULL (the
+ * shifted cell coordinates are the positions), all device.  Writes the
+ * selected level per point to out_level (int32[n], device, nullable) and
+ * the statistics block stats (int64[FK_ADAPTIVE_STATS], device): [0] a
+ * device-side status (FK_ERR_DOMAIN), [1] its reason (1 batch index, 2
+ * span), [2] the largest shifted coordinate, [3] the coarsest level, then
+ * per level s = 0..16 at [4 + 4s]: points that selected s and the sum, min
+ * and max of their counts.
+ *
+ * fk_adaptive_query: the k nearest (fp32 squared distance, ties to the
+ * smaller index, d < cut_r2) among the points of the same batch in the 27
+ * nodes around each point's node at its selected level (:369-488); rows in
+ * input order, int64 out_idx[n,k], float out_dist[n,k] (nullable); slots
+ * without a candidate hold (the point's own index, +inf).  Reads the tables
+ * fk_adaptive_select left in the workspace; writes nothing when stats[0]
+ * holds an error.
+ *
+ * workspace (device) must hold fk_adaptive_workspace_bytes(n) bytes.
+ * k outside [1, 64] returns FK_ERR_K_RANGE. */
+#define FK_ADAPTIVE_STATS 72
+int64_t fk_adaptive_workspace_bytes(int64_t n);
+int fk_adaptive_select(const int64_t *grid_coord, const int64_t *batch_idx,
+                       const float *xyz, int64_t n, int32_t k,
+                       int32_t min_factor, int32_t max_factor,
+                       int32_t max_shift, void *workspace, int32_t *out_level,
+                       int64_t *stats, void *stream);
+int fk_adaptive_query(void *workspace, int64_t n, int32_t k, float cut_r2,
+                      const int64_t *stats, int64_t *out_idx, float *out_dist,
+                      void *stream);
 
 /* -------------------------------------------- voxel-first subsampling */
 /* Replaces _voxel_first_cuda (Networks/hierarchy.py:23-36), the step that
