@@ -81,8 +81,8 @@ namespace {
 //                    with a half-cleaner) instead of KC
 //   FK_WPE_<KC>      minimum waves per SIMD requested for the first pass
 //                    (3 for K class 16 holds it to 168 VGPRs, three waves
-//                    per SIMD, at a 12-byte spill: faster than 171 VGPRs at
-//                    two)
+//                    per SIMD: the planned kernel fits, the uniform one
+//                    spills a few bytes; faster than 171 VGPRs at two)
 //   FK_SLOTS         records per lane per round of the K class 8/16 stream
 //   FK_FILL_WAVES    a launch with fewer waves widens its lane groups
 #ifndef FK_UNROLL
