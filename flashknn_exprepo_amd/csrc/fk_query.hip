@@ -29,8 +29,9 @@
 // block (faces beyond which the cloud has no cells do not count), no point
 // outside the block can enter, and the result equals brute force.  The rest
 // go to a pending list and a second kernel grows their block shell by shell
-// (radius from the current K-th distance, doubled while it is infinite)
-// until they certify.
+// (radius from the current K-th distance; while there is none, from a probe
+// of full-height block counts, else grown geometrically) until they
+// certify.
 //
 // The file is compiled twice: with FK_COLUMNS=0 for grids with a dense cell
 // table, and with FK_COLUMNS=1 for the column layout (column_range searches
@@ -1442,6 +1443,83 @@ __device__ __forceinline__ float coop_eff(const CoopList<P> &t, float cut_r2) {
   return e < cut_r2 ? e : cut_r2;
 }
 
+// Sum over the wave (every lane active), by DPP and row swaps.
+__device__ __forceinline__ int wave_sum(int v) {
+  v += (int)lane_xor32<1>((uint32_t)v);
+  v += (int)lane_xor32<2>((uint32_t)v);
+  v += (int)lane_xor32<4>((uint32_t)v);
+  v += (int)lane_xor32<8>((uint32_t)v);
+  v += (int)lane_xor32<16>((uint32_t)v);
+  v += (int)lane_xor32<32>((uint32_t)v);
+  return v;
+}
+
+// Half-widths the radius probe tries (at most 31: one lane per row), in
+// two rounds of loads (the second only when the first finds nothing).
+constexpr int kProbeRadii[] = {2, 3, 4, 6, 8, 11, 16, 22, 31};
+constexpr int kNumProbes = sizeof(kProbeRadii) / sizeof(kProbeRadii[0]);
+constexpr int kProbeSplit = 4;
+
+// Probes J0 .. J1 - 1 (see probe_radius); 0 if none qualifies.
+template <int J0, int J1>
+__device__ __forceinline__ int probe_round(const CloudInfo &ci, const int32_t *__restrict__ cs,
+                                           int4 qc, int Ro, int Rcover, int k, int dx) {
+  const int x = qc.x + dx;
+  const bool row = x >= 0 && x < ci.nx;
+  int cnt[J1 - J0];
+#pragma unroll
+  for (int j = J0; j < J1; ++j) {
+    const int R = kProbeRadii[j] < Rcover ? kProbeRadii[j] : Rcover;
+    cnt[j - J0] = 0;
+    if (R > Ro && row && dx >= -R && dx <= R) {
+      const int y0 = max(qc.y - R, 0), y1 = min(qc.y + R, ci.ny - 1);
+      if (y0 <= y1) cnt[j - J0] = cs[lin_of(ci, x, y1 + 1, 0)] - cs[lin_of(ci, x, y0, 0)];
+    }
+  }
+  int best = 0;
+#pragma unroll
+  for (int j = J1 - 1; j >= J0; --j) {
+    const int R = kProbeRadii[j] < Rcover ? kProbeRadii[j] : Rcover;
+    if (R > Ro && wave_sum(cnt[j - J0]) >= k) best = R;
+  }
+  return best;
+}
+
+// Wave-uniform.  For a query that holds fewer than k candidates (no k-th
+// distance to size the next block): the smallest probe half-width R above
+// Ro (clipped to Rcover) whose block holds at least k points when counted
+// over the full height of the box, or 0 if none does.  The count of
+// columns (x, y0..y1) over all z is the difference of two cell-table
+// entries (a row's cells are consecutive, z fastest; the column layout has
+// one entry per column), so lane l sums row x = qx + l - 32 for several
+// probes at once: one or two rounds of loads instead of the shell-by-shell
+// growth through empty space around an isolated query (a far LiDAR return
+// needed up to nine shells to reach its neighbours 8 m away).  The count
+// is an upper bound of the block's, so the block may still hold fewer
+// than k; the growth then goes on from it.
+__device__ __forceinline__ int probe_radius(const CloudInfo &ci, const int32_t *__restrict__ cs,
+                                            int4 qc, int Ro, int Rcover, int k, int lane) {
+  const int dx = lane - 32;
+  const int r = probe_round<0, kProbeSplit>(ci, cs, qc, Ro, Rcover, k, dx);
+  return r ? r : probe_round<kProbeSplit, kNumProbes>(ci, cs, qc, Ro, Rcover, k, dx);
+}
+
+// The next half-width of a cooperative list's block (wave-uniform): from
+// the k-th distance (or the cut) when there is one, else from the radius
+// probe, else by the growth rule of next_radius.  The results never depend
+// on it: every shell sequence ends with the same certified block contents.
+template <int P>
+__device__ __forceinline__ int coop_next_radius(const CoopList<P> &t, const QueryArgs &a,
+                                                const CloudInfo &ci, int4 qc, float4 qr,
+                                                int Ro, int Rcover, int lane) {
+  const float eff = coop_eff(t, a.cut_r2);
+  if (!(eff < INFINITY)) {
+    const int Rp = probe_radius(ci, a.cell_start, qc, Ro, Rcover, t.k, lane);
+    if (Rp > Ro) return Rp;
+  }
+  return next_radius(ci, qc, qr, Ro, eff, Rcover, coop_found(t, lane), t.k);
+}
+
 // Grow the block from half-width Ro shell by shell until the query
 // certifies or the block covers the cloud's box.  Returns the final Ro.
 template <int P>
@@ -1485,11 +1563,12 @@ __global__ void __launch_bounds__(64 * kCoopWaves, 4) k_expand_coop(QueryArgs a)
     t.cutkey = make_key(a.cut_r2, 0u);
     t.L[0] = lane < k ? a.pend_keys[(int64_t)slot * a.k + lane] : kMaxKey;
     t.refresh(lane);
-    // the first pass left it uncertified at half-width a.R
+    // the first pass left it uncertified at half-width a.R; the first
+    // shell is sized by the radius probe when the list is short of k (once:
+    // inside the growth loop the probe's registers spilled the kernel)
     const int Rcover = cover_radius(ci, qc);
     if (a.R < Rcover) {
-      const int Rn = next_radius(ci, qc, qr, a.R, coop_eff(t, a.cut_r2), Rcover,
-                                 coop_found(t, lane), k);
+      const int Rn = coop_next_radius<1>(t, a, ci, qc, qr, a.R, Rcover, lane);
       coop_shell<1>(t, a, ci, qc, qr, a.R, Rn, lane, wave_tab);
       coop_expand<1>(t, a, ci, qc, qr, Rn, lane, wave_tab);
     }

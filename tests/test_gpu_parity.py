@@ -663,7 +663,7 @@ def test_idw_interp_padding_and_channels(oracle):
 
 
 def test_idw_fp64_finite_differences(oracle):
-    """C5 check on a 2k-point copy: gradients of the fp32 path vs FP64
+    """Backward check on a 2k-point copy: gradients of the fp32 path vs FP64
     central differences of the same loss, relative error <= 1e-5."""
     from flashknn_exprepo_amd import synthetic
     fine = synthetic.room(seed=51, target=20000)
@@ -812,6 +812,36 @@ def test_exact_build_explicit_cell_beyond_reserve(oracle):
     idx, d = eng.grid.query(8, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
     eng.grid.status()
     _same(idx, d, ref_i, ref_d)
+
+
+@pytest.mark.parametrize("columns", [False, True])
+@pytest.mark.parametrize("k", [4, 16, 32])
+def test_exact_isolated_queries_radius_probe(oracle, k, columns):
+    """Queries short of k candidates after the first pass take their first
+    expansion shell from the full-height block-count probe: isolated points
+    2-14 m from a dense floor patch (jumps of up to 31 cells), points in
+    the box corners (probe radii clipped to the cover radius), and a tall
+    stack of points above an isolated query (the full-height count then
+    overstates the block, so the growth goes on after the jump); exact
+    against brute force on the dense and the column layout."""
+    from flashknn_exprepo_amd import _lib as F
+    rng = np.random.default_rng(31)
+    floor = np.c_[rng.uniform(0, 4, (3000, 2)), rng.normal(0, 0.01, 3000)]
+    far = np.c_[rng.uniform(-14, 18, (60, 2)), rng.uniform(-0.5, 3, 60)]
+    corners = np.array([[-14, -14, -0.5], [18, 18, 3], [18, -14, 3], [-14, 18, -0.5]])
+    stack = np.c_[np.full((40, 2), [9.0, -6.0]) + rng.normal(0, 0.02, (40, 2)),
+                  np.linspace(0.5, 3.0, 40)]
+    s = np.concatenate([floor, far, corners, stack, [[9.2, -6.1, -0.4]]]).astype(np.float32)
+    n = len(s)
+    ref_i, ref_d = oracle.brute_knn(s, [n], s, [n], k)
+    from flashknn_exprepo_amd.grid import Grid
+    g = Grid().build_xyz(_t(s), _t(np.array([n], np.int32)),
+                         torch.tensor([1 / 0.1], dtype=torch.float32, device=_dev()),
+                         columns=columns)
+    idx, d = g.query(k, mode=F.FK_MODE_EXACT, pad_mode=F.FK_PAD_MINUS1)
+    g.status()
+    _same(idx, d, ref_i, ref_d)
+    assert g.query_stats()["expanded"] >= 60
 
 
 @pytest.mark.parametrize("k", [8, 16, 100])
