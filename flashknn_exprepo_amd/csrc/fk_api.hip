@@ -688,6 +688,7 @@ static int make_args(fk_grid *g, const fk_query_params *p, void *out_idx,
   a.scalars = g->scalars;
   a.qcnt = g->scalars + kCounters + kCntStride * g->qset;
   a.qcnt_next = nullptr;
+  a.pend_class = -1;
   return FK_OK;
 }
 
@@ -759,12 +760,18 @@ int fk_query_stats(fk_grid *g, int64_t *first_pass_certified, int64_t *expanded,
                    void *stream) {
   FK_CHECK(check_stream_device(g));
   FK_CHECK(read_scalars(g, S(stream)));
-  const int32_t *c = g->h_pinned + kCounters + kCntStride * g->stat_set;
-  const int64_t inl = c[kCntInline] < kInlineBudget ? c[kCntInline] : kInlineBudget;
-  const int64_t listed = (int64_t)c[kCntPending] + inl;
+  // a batched launch keeps one set per K class
+  const int sets = g->stat_set == kBatchedSet ? kBatchedClasses : 1;
+  int64_t listed = 0, exp = 0;
+  for (int i = 0; i < sets; ++i) {
+    const int32_t *c = g->h_pinned + kCounters + kCntStride * (g->stat_set + i);
+    const int64_t inl = c[kCntInline] < kInlineBudget ? c[kCntInline] : kInlineBudget;
+    listed += (int64_t)c[kCntPending] + inl;
+    exp += c[kCntExpanded];
+  }
   const int64_t pending = listed < g->stat_nq ? listed : g->stat_nq;
   if (first_pass_certified) *first_pass_certified = g->stat_nq - pending;
-  if (expanded) *expanded = c[kCntExpanded];
+  if (expanded) *expanded = exp;
   return ok();
 }
 
@@ -898,18 +905,19 @@ static int batched_args(fk_grid *g, const fk_query_params *p,
   }
   a.k_per_cloud = k_per_cloud;
   a.out_offsets = out_offsets;
-  a.qcnt = g->scalars + kCounters + kCntStride * kBatchedSet;  // cleared by k_items
+  // class 0's counter set; class c uses set kBatchedSet + c (cleared by k_items)
+  a.qcnt = g->scalars + kCounters + kCntStride * kBatchedSet;
   a.qcnt_next = nullptr;
   a.num_clouds = g->n_clouds;
   a.item_cloud = g->item_prefix;
   return FK_OK;
 }
 
-// Work items, then one persistent first-pass kernel per K class up to
-// max_k's (the classes above the first forked onto the handle's auxiliary
-// streams, so that one class's last tiles overlap the others' instead of
-// leaving the GPU idle), joined back on s, then the shell expansion of the
-// pending queries of all classes.  Graph-capturable (the fork and join are
+// Work items, then per K class up to max_k's one persistent first-pass
+// kernel and the shell expansion of that class's pending list (the classes
+// above the first forked onto the handle's auxiliary streams, so that one
+// class's last tiles and expansion overlap the others' instead of leaving
+// the GPU idle), joined back on s.  Graph-capturable (the fork and join are
 // events).
 #ifndef FK_BATCHED_FORK
 #define FK_BATCHED_FORK 1
@@ -927,6 +935,21 @@ static int ensure_fork(fk_grid *g) {
   return FK_OK;
 }
 
+// One K class: its first pass, then the expansion of its own pending list
+// (so that a class that finishes early expands while the others still run).
+static QueryArgs class_args(const QueryArgs &a, int cls) {
+  QueryArgs c = a;
+  c.qcnt = a.qcnt + kCntStride * cls;
+  c.pend_class = cls;
+  return c;
+}
+static int enqueue_class(const QueryArgs &a, int cls, hipStream_t s) {
+  const QueryArgs c = class_args(a, cls);
+  FK_HIP(launch_batched_first(c, cls, s));
+  FK_HIP(c.columns ? launch_expand_columns(c, s) : launch_expand(c, s));
+  return FK_OK;
+}
+
 static int enqueue_batched(fk_grid *g, const QueryArgs &a, int32_t max_k, hipStream_t s) {
   FK_HIP(launch_items(g->clouds, g->cell_start, a.k_per_cloud, g->n_clouds, max_k,
                       g->item_prefix, g->scalars, a.qcnt, s));
@@ -936,15 +959,14 @@ static int enqueue_batched(fk_grid *g, const QueryArgs &a, int32_t max_k, hipStr
     for (int cls = top; cls >= 1; --cls) {
       hipStream_t as = g->aux[cls - 1];
       FK_HIP(hipStreamWaitEvent(as, g->ev_fork, 0));
-      FK_HIP(launch_batched_first(a, cls, as));
+      FK_CHECK(enqueue_class(a, cls, as));
       FK_HIP(hipEventRecord(g->ev_join[cls - 1], as));
     }
-    FK_HIP(launch_batched_first(a, 0, s));
+    FK_CHECK(enqueue_class(a, 0, s));
     for (int cls = top; cls >= 1; --cls) FK_HIP(hipStreamWaitEvent(s, g->ev_join[cls - 1], 0));
   } else {
-    for (int cls = top; cls >= 0; --cls) FK_HIP(launch_batched_first(a, cls, s));
+    for (int cls = top; cls >= 0; --cls) FK_CHECK(enqueue_class(a, cls, s));
   }
-  FK_HIP(a.columns ? launch_expand_columns(a, s) : launch_expand(a, s));
   return FK_OK;
 }
 

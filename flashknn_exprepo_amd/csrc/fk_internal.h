@@ -24,22 +24,26 @@ enum Scalar : int {
   kTotalCells = 1,    // cells of all per-cloud boxes
   kMaxDepth = 2,      // column layouts: largest box depth nz of a cloud
   kQueueHead = 3,     // batched mode: next work item of K class c at kQueueHead + c
+  kPendBase = 7,      // batched mode: first pending-list slot of K class c at kPendBase + c
   kCounters = 16,     // query counter sets: kCounters + kCntStride * set
-  kNumScalars = 32
+  kNumScalars = 48
 };
 
 // One set of query counters.  Plain queries alternate between sets 0 and 1
 // (each first pass clears the other set for the next call, so no separate
-// reset launch is needed); batched launches use set 2, cleared by k_items.
-// (The certified count is not kept: it is the query count minus the
-// pending one.)
+// reset launch is needed); batched launches use sets 2 .. 5, one per K
+// class, cleared by k_items.  (The certified count is not kept: it is the
+// query count minus the pending one.)
 enum Counter : int {
   kCntPending = 0,    // exact mode: queries left uncertified by pass 1
   kCntInline = 1,     // exact mode: uncertified ones the first pass expanded itself
   kCntExpanded = 2,   // exact mode: resolved by shell expansion (both kinds)
   kCntStride = 4      // slots per set (slot 3 spare)
 };
-constexpr int kBatchedSet = 2;
+constexpr int kBatchedSet = 2;  // K class c of a batched launch: set kBatchedSet + c
+static_assert(kPendBase + kBatchedClasses <= kCounters, "scalar slots");
+static_assert(kCounters + kCntStride * (kBatchedSet + kBatchedClasses) <= kNumScalars,
+              "counter sets");
 // Uncertified queries a plain first pass may expand itself per launch
 // (fk_query.hip, run_tile); every claim counts in kCntInline, those past the
 // budget are listed for the expansion launch instead.
@@ -454,6 +458,10 @@ struct QueryArgs {
   int32_t *qcnt_next;    // set the first pass clears for the next call (or null)
   int32_t *pend_q;       // [nq]
   uint64_t *pend_keys;   // [nq * k]
+  // batched mode: the K class whose pending list (from slot
+  // scalars[kPendBase + pend_class] on) this launch appends to or expands;
+  // -1: the list starts at slot 0
+  int32_t pend_class;
   // batched mode
   const int32_t *k_per_cloud;
   const int64_t *out_offsets;

@@ -967,6 +967,13 @@ __device__ __forceinline__ void inline_expand(const QueryArgs &a, int q, int lan
 // batched kernel.  With INL (the plain kernel) it returns the sorted index of
 // a query left for the wave to expand (its list in the wave's seed area), or
 // -1.
+// First pending-list slot of the launch's list (per K class in batched
+// mode, written by k_items); the first-pass kernels take lists already
+// offset (pend_class -1).
+__device__ __forceinline__ int pend_base(const QueryArgs &a) {
+  return a.pend_class >= 0 ? a.scalars[kPendBase + a.pend_class] : 0;
+}
+
 template <int KC, int G, bool INL = false, int CB = default_batch<KC>()>
 __device__ __forceinline__ int run_tile(const QueryArgs &a, int q0, int qend,
                                         uint64_t *lane_buf, uint64_t *lane_tab) {
@@ -1545,12 +1552,14 @@ __global__ void __launch_bounds__(64 * kCoopWaves, 4) k_expand_coop(QueryArgs a)
   __shared__ uint64_t ctab[kCoopWaves * kShellTab];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
   uint64_t *wave_tab = ctab + w * kShellTab;
-  const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq);
+  const int base = pend_base(a);
+  const int npend = a.scalars[kStatus] != 0 ? 0 : min(a.qcnt[kCntPending], a.nq - base);
   int nexp = 0;
   // static striding over the pending list (a shared work counter measured
   // slower: its atomics cost more than the balance they buy)
   const int nwaves = gridDim.x * kCoopWaves;
-  for (int slot = blockIdx.x * kCoopWaves + w; slot < npend; slot += nwaves) {
+  for (int i = blockIdx.x * kCoopWaves + w; i < npend; i += nwaves) {
+    const int slot = base + i;
     const int qs = a.pend_q[slot];
     const float4 qr = a.qrec[qs];
     const int4 qc = a.qcell[qs];
@@ -1724,6 +1733,16 @@ k_batched(QueryArgs a, int cls) {
   const int32_t *prefix = a.item_cloud + cls * (a.num_clouds + 1);
   int32_t *head = a.scalars + kQueueHead + cls;
   const int nitems = a.scalars[kStatus] != 0 ? 0 : prefix[a.num_clouds];
+  // the class's pending list, located once per wave (a read of the base
+  // per tile, next to the queue heads' atomics, tripled the kernel's time)
+  QueryArgs b = a;
+  if (a.pend_class >= 0) {
+    const int base = a.scalars[kPendBase + a.pend_class];
+    b.pend_q = a.pend_q + base;
+    b.pend_keys = a.pend_keys + (int64_t)base * a.k;
+    b.nq = a.nq - base;
+    b.pend_class = -1;
+  }
   for (;;) {
     int item = 0;
     if (lane == 0) item = atomicAdd(head, 1);
@@ -1734,21 +1753,25 @@ k_batched(QueryArgs a, int cls) {
     const int qb = a.cell_start[ci.base];
     const int qe = a.cell_start[ci.base + table_entries(ci)];
     const int q0 = qb + (item - prefix[c]) * (kWave / G);
-    run_tile<KC, G>(a, q0, qe, lane_buf, lane_tab);
+    run_tile<KC, G>(b, q0, qe, lane_buf, lane_tab);
   }
 }
 
 #if !FK_COLUMNS && FK_COMMON
 // Per-class tile counts -> one exclusive prefix per K class over the clouds
 // (one block): prefix[cls * (B + 1) + c] is cloud c's first item in class
-// cls's queue (clouds of other classes contribute none).  A cloud whose k
-// lies outside [1, max_k] raises FK_ERR_K_RANGE, and then no tile runs.
+// cls's queue (clouds of other classes contribute none).  Each class's
+// pending list starts at scalars[kPendBase + cls], after the queries of
+// the classes below it.  A cloud whose k lies outside [1, max_k] raises
+// FK_ERR_K_RANGE, and then no tile runs.  qcnt: the counter set of class 0
+// (the classes' sets follow it), cleared here.
 __global__ void __launch_bounds__(1024)
 k_items(const CloudInfo *clouds, const int32_t *cell_start, const int32_t *k_per_cloud,
         int32_t B, int32_t max_k, int32_t *prefix, int32_t *scalars, int32_t *qcnt) {
   __shared__ int wsum[kBatchedClasses][16];
   __shared__ int carry[kBatchedClasses];
-  if (threadIdx.x < kBatchedClasses) carry[threadIdx.x] = 0;
+  __shared__ int queries[kBatchedClasses];
+  if (threadIdx.x < kBatchedClasses) carry[threadIdx.x] = queries[threadIdx.x] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   // After a build that raised an error (an overflowing no-sync build still
@@ -1767,6 +1790,7 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, const int32_t *k_per
         cls = batched_class(k);
         const int tile = batched_tile(cls);
         tiles = (n + tile - 1) / tile;
+        if (n > 0) atomicAdd(&queries[cls], n);
       }
     }
     int v[kBatchedClasses];
@@ -1798,11 +1822,11 @@ k_items(const CloudInfo *clouds, const int32_t *cell_start, const int32_t *k_per
   if (threadIdx.x < kBatchedClasses) {
     prefix[threadIdx.x * (B + 1) + B] = carry[threadIdx.x];
     scalars[kQueueHead + threadIdx.x] = 0;
+    int below = 0;
+    for (int j = 0; j < (int)threadIdx.x; ++j) below += queries[j];
+    scalars[kPendBase + threadIdx.x] = below;
   }
-  if (threadIdx.x == 0) {
-    qcnt[kCntPending] = 0;
-    qcnt[kCntExpanded] = 0;
-  }
+  if (threadIdx.x < kCntStride * kBatchedClasses) qcnt[threadIdx.x] = 0;
 }
 #endif
 

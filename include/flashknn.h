@@ -296,11 +296,11 @@ int fk_point_candidates(fk_grid *grid, int32_t radius_cells, int32_t query_set,
 
 /* ------------------------------------------- batched persistent query */
 /* All clouds of a grid built by fk_grid_build_xyz or fk_grid_build_cells,
- * with a per-cloud k, in one call: one tile-count launch, then one
- * persistent kernel per K class present (8 / 16 / 32 / 64, each with its
- * own register budget and tile queue; classes forked onto the handle's
- * auxiliary streams and joined back onto `stream`), then one shared
- * expansion of the uncertified queries.
+ * with a per-cloud k, in one call: one tile-count launch, then per K class
+ * up to params->k's (8 / 16 / 32 / 64) one persistent kernel with its own
+ * register budget and tile queue followed by the expansion of that class's
+ * uncertified queries; the classes are forked onto the handle's auxiliary
+ * streams and joined back onto `stream`.
  * k_per_cloud: int32[num_clouds] (device).  out_offsets: int64[num_clouds]
  * (device) start of cloud c's rows in out_idx/out_dist, in elements
  * (row r of cloud c lives at out_offsets[c] + (r - first point of c) * k_c).
