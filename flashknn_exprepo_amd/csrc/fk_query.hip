@@ -1702,15 +1702,27 @@ __device__ __forceinline__ int item_to_cloud(const int32_t *prefix, int B,
 }
 
 // The batched launch's K classes (k up to 8, 16, 32, 64 -> 0..3) and
-// their work items (kWave / G queries of one cloud).
+// their work items (kWave / G queries of one cloud).  The batched kernels
+// take one lane per query (FK_BATCHED_G): a batch holds many more tiles
+// than wave slots, so per-query cost, not a wave's latency, sets the step,
+// and one lane per query is the cheapest per query (16-cloud batch: 0.519
+// ms with two lanes for K classes 8/16, 0.490 ms with one; K class 32
+// alone 0.509).
+#ifndef FK_BATCHED_G
+#define FK_BATCHED_G 1
+#endif
+template <int KC>
+__host__ __device__ constexpr int batched_lanes() {
+  return KC == 64 ? lanes_per_query<64>() : FK_BATCHED_G;
+}
 __host__ __device__ constexpr int batched_class(int k) {
   return k <= 8 ? 0 : (k <= 16 ? 1 : (k <= 32 ? 2 : 3));
 }
 __host__ __device__ constexpr int batched_tile(int cls) {
-  return kWave / (cls == 0   ? lanes_per_query<8>()
-                  : cls == 1 ? lanes_per_query<16>()
-                  : cls == 2 ? lanes_per_query<32>()
-                             : lanes_per_query<64>());
+  return kWave / (cls == 0   ? batched_lanes<8>()
+                  : cls == 1 ? batched_lanes<16>()
+                  : cls == 2 ? batched_lanes<32>()
+                             : batched_lanes<64>());
 }
 static_assert(batched_class(64) == kBatchedClasses - 1, "batched classes");
 
@@ -1735,7 +1747,7 @@ __global__ void __launch_bounds__(64 * waves_per_block<KC>(), min_waves_per_eu<K
 k_batched(QueryArgs a, int cls) {
   constexpr int WPB = waves_per_block<KC>();
   constexpr int BS = buffer_slots<KC>();
-  constexpr int G = lanes_per_query<KC>();
+  constexpr int G = batched_lanes<KC>();
   __shared__ uint64_t sbuf[WPB * BS * kWave];
   __shared__ uint64_t stab[WPB * stream_entries<KC>() * kWave];
   const int w = threadIdx.x >> 6, lane = threadIdx.x & (kWave - 1);
