@@ -147,6 +147,7 @@ constexpr int lanes_per_query() {
   return KC == 8 ? FK_G_8 : KC == 16 ? FK_G_16 : KC == 32 ? FK_G_32 : FK_G_64;
 }
 
+
 template <int KC>
 constexpr int min_waves_per_eu() {
   return KC == 8 ? FK_WPE_8 : KC == 16 ? FK_WPE_16 : KC == 32 ? FK_WPE_32 : FK_WPE_64;
@@ -1087,7 +1088,9 @@ k_query_planned(QueryArgs a) {
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
-  const int wid = blockIdx.x * WPB + w;
+  const int blk = plan_xcd_region(a.plan_waves) > 0 ? xcd_remap(blockIdx.x, gridDim.x)
+                                                    : (int)blockIdx.x;
+  const int wid = blk * WPB + w;
   if (wid >= a.plan_waves || a.scalars[kStatus] != 0) return;
   const int2 d = a.plan[wid];
   const int q0 = __builtin_amdgcn_readfirstlane(d.x);
@@ -1729,12 +1732,22 @@ static_assert(batched_class(64) == kBatchedClasses - 1, "batched classes");
 // Candidate batch of the batched first pass: the default of the K class,
 // the large one for K class 32 (its LiDAR and room clouds ran 8 % faster
 // with it alone, the 16-cloud batch 1 %).
-#ifndef FK_BATCHED_LARGE32
-#define FK_BATCHED_LARGE32 1
+// FK_BATCHED_CB_<KC>: the BatchSel of each class's batched kernel.
+#ifndef FK_BATCHED_CB_8
+#define FK_BATCHED_CB_8 0
+#endif
+#ifndef FK_BATCHED_CB_16
+#define FK_BATCHED_CB_16 0
+#endif
+#ifndef FK_BATCHED_CB_32
+#define FK_BATCHED_CB_32 2
 #endif
 template <int KC>
 constexpr int batched_records() {
-  return KC == 32 && FK_BATCHED_LARGE32 ? batch_records<32>(kBatchLarge) : default_batch<KC>();
+  return KC == 8    ? batch_records<8>(FK_BATCHED_CB_8)
+         : KC == 16 ? batch_records<16>(FK_BATCHED_CB_16)
+         : KC == 32 ? batch_records<32>(FK_BATCHED_CB_32)
+                    : default_batch<KC>();
 }
 
 // Batched first pass of one K class: a persistent kernel whose waves pull
@@ -1915,6 +1928,7 @@ __global__ void __launch_bounds__(256) k_tile_cost(const QueryArgs a, int32_t *t
 
 constexpr int kPlanMaxTiles = 8192;  // super-tiles cached in LDS by k_plan
 
+
 constexpr int kPlanSearch = 7;       // bisection steps over the 2^7 budgets
 
 // One lane per query (64 queries per wave) measured about 1.4x slower per
@@ -1943,13 +1957,26 @@ __device__ __forceinline__ float plan_budget(int j) { return 4.f * exp2f(0.125f 
 // per-wave plan, super-tiles with more lanes first (they are the longest
 // waves: started first, they finish with the rest).  Unused plan entries
 // get (nq, 1) and exit at once.
+//
+// With region > 0 (capacity / 8; see plan_xcd_region) the plan is laid out
+// per XCD: entries [x * region, (x + 1) * region) are the waves that XCD x
+// runs (k_query_planned remaps its blocks with xcd_remap).  Each lane class
+// is cut into kPlanChunks runs of consecutive super-tiles and run j goes to
+// XCD j % 8, so every XCD gets the same mix of classes and kPlanChunks / 8
+// slabs of the cell-sorted queries spread over the cloud: its L2 fetches
+// the records of those slabs instead of most of the cloud (dealing single
+// waves round-robin over the XCDs spreads every slab over all eight), and
+// the slabs sample the cloud's dense and sparse parts alike (one slab per
+// XCD left some XCDs with more of the heavy waves).  The budget is the
+// smallest whose largest XCD share fits the region.
 #if FK_COMMON
 __global__ void __launch_bounds__(1024)
 k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
-       const int32_t *scalars, int2 *plan) {
+       int32_t region, const int32_t *scalars, int2 *plan) {
   __shared__ int cost[kPlanMaxTiles];
   __shared__ int red[16];
   __shared__ int cls_n[4], carry[4];
+  __shared__ int xoff[8][4], xused[8], fits;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const bool ok = scalars[kStatus] == 0;
   if (!ok) {
@@ -1968,32 +1995,71 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     __syncthreads();
     return s;
   };
+  // super-tiles per class (G = 8, 4, 2, 1) under budget L -> cls_n
+  auto class_counts = [&](float L) {
+    for (int c = 0; c < 4; ++c) {
+      int v = 0;
+      for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L) == (8 >> c);
+      const int n = block_sum(v);
+      if (t == 0) cls_n[c] = n;
+    }
+    __syncthreads();
+  };
+  // class c's super-tiles in kPlanChunks runs; run j goes to XCD j % 8
+  auto csize = [](int n, int j) { return (j + 1) * n / kPlanChunks - j * n / kPlanChunks; };
+  auto share = [&](int n, int x) {  // XCD x's super-tiles of a class of n
+    int v = 0;
+    for (int j = x; j < kPlanChunks; j += 8) v += csize(n, j);
+    return v;
+  };
   // waves(L) does not grow with L: bisect the budget levels
   int lo = 0, hi = (1 << kPlanSearch) - 1;  // hi always fits (8192 per lane)
   for (int it = 0; it < kPlanSearch; ++it) {
     const int mid = (lo + hi) >> 1;
     const float L = plan_budget(mid);
-    int v = 0;
-    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L);
-    if (block_sum(v) <= capacity) hi = mid; else lo = mid + 1;
+    bool fit;
+    if (region > 0) {
+      class_counts(L);
+      if (t == 0) {
+        int worst = 0;
+        for (int x = 0; x < 8; ++x) {
+          int v = 0;
+          for (int c = 0; c < 4; ++c) v += share(cls_n[c], x) * (8 >> c);
+          worst = max(worst, v);
+        }
+        fits = worst <= region;
+      }
+      __syncthreads();
+      fit = fits != 0;
+      __syncthreads();
+    } else {
+      int v = 0;
+      for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L);
+      fit = block_sum(v) <= capacity;
+    }
+    if (fit) hi = mid; else lo = mid + 1;
   }
   // the last level puts every super-tile on kPlanGmin lanes per query
-  // (waves = kPlanGmin * super-tiles <= capacity, which the host checks)
+  // (waves = kPlanGmin * super-tiles <= capacity, which the host checks;
+  // per XCD at most ceil(ntiles / 8) <= region, checked by plan_xcd_region)
   const float L = hi == (1 << kPlanSearch) - 1 ? INFINITY : plan_budget(hi);
-  // per class (G = 8, 4, 2, 1): super-tile count, then first wave of the class
-  for (int c = 0; c < 4; ++c) {
-    int v = 0;
-    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L) == (8 >> c);
-    const int n = block_sum(v);
-    if (t == 0) cls_n[c] = n;
-  }
+  class_counts(L);
   if (t == 0) {
     int base = 0;
     for (int c = 0; c < 4; ++c) {
       const int n = cls_n[c];
-      carry[c] = base;  // running first wave of the next super-tile of class c
+      carry[c] = region > 0 ? 0 : base;  // region: rank of the next super-tile of class c
       base += n * (8 >> c);
     }
+    if (region > 0)
+      for (int x = 0; x < 8; ++x) {
+        int o = x * region;
+        for (int c = 0; c < 4; ++c) {
+          xoff[x][c] = o;
+          o += share(cls_n[c], x) * (8 >> c);
+        }
+        xused[x] = o;
+      }
   }
   __syncthreads();
   __shared__ int wcnt[4][16];
@@ -2011,7 +2077,16 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     if (g != 0) {
       int before = 0;  // super-tiles of class c ahead of this one in the chunk
       for (int j = 0; j < w; ++j) before += wcnt[c][j];
-      const int first = carry[c] + (before + pos) * g;
+      int first;
+      if (region > 0) {
+        const int n = cls_n[c], r = carry[c] + before + pos;  // rank in class c
+        const int j = (kPlanChunks * (r + 1) + n - 1) / n - 1;  // its run: start <= r
+        int o = r - j * n / kPlanChunks;  // ahead of it in XCD j % 8's share
+        for (int jj = j & 7; jj < j; jj += 8) o += csize(n, jj);
+        first = xoff[j & 7][c] + o * g;
+      } else {
+        first = carry[c] + (before + pos) * g;
+      }
       for (int p = 0; p < g; ++p)
         if (first + p < capacity)  // always (the waves fit by construction)
           plan[first + p] = make_int2(i * kPlanTile + p * (kPlanTile / g), g);
@@ -2020,9 +2095,14 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     if (t < 4) {
       int tot = 0;
       for (int j = 0; j < 16; ++j) tot += wcnt[t][j];
-      carry[t] += tot * (8 >> t);
+      carry[t] += region > 0 ? tot : tot * (8 >> t);
     }
     __syncthreads();
+  }
+  if (region > 0) {
+    for (int x = 0; x < 8; ++x)
+      for (int i = xused[x] + t; i < (x + 1) * region; i += 1024) plan[i] = make_int2(nq, 1);
+    return;
   }
   int used = 0;
   for (int c = 0; c < 4; ++c) used += cls_n[c] * (8 >> c);
@@ -2373,11 +2453,11 @@ hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
                                   int32_t *tile_cost, int2 *plan, hipStream_t s) {
   FK_FORWARD(launch_plan_columns(a, capacity, tile_cost, plan, s));
   const int64_t ntiles = ((int64_t)a.nq + kPlanTile - 1) / kPlanTile;
-  if (a.nq <= 0 || ntiles * kPlanGmin > capacity || ntiles > kPlanMaxTiles)
+  if (a.nq <= 0 || !plan_fits(ntiles * kPlanGmin, capacity) || ntiles > kPlanMaxTiles)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_tile_cost, dim3((a.nq + 255) / 256), dim3(256), 0, s, a, tile_cost);
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, tile_cost, (int32_t)ntiles, a.nq,
-                     capacity, a.scalars, plan);
+                     capacity, plan_xcd_region(capacity), a.scalars, plan);
   return hipGetLastError();
 }
 
