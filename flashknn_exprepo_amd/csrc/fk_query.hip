@@ -1995,13 +1995,22 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     __syncthreads();
     return s;
   };
-  // super-tiles per class (G = 8, 4, 2, 1) under budget L -> cls_n
+  // super-tiles per class (G = 8, 4, 2, 1) under budget L -> cls_n: the
+  // four counts in 16-bit fields of two sums (ntiles <= kPlanMaxTiles)
+  static_assert(kPlanMaxTiles < (1 << 16), "16-bit class counts");
   auto class_counts = [&](float L) {
-    for (int c = 0; c < 4; ++c) {
-      int v = 0;
-      for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L) == (8 >> c);
-      const int n = block_sum(v);
-      if (t == 0) cls_n[c] = n;
+    int v01 = 0, v23 = 0;
+    for (int i = t; i < ntiles; i += 1024) {
+      const int g = plan_lanes(cost[i], L);
+      v01 += g == 8 ? 1 : (g == 4 ? 1 << 16 : 0);
+      v23 += g == 2 ? 1 : (g == 1 ? 1 << 16 : 0);
+    }
+    const int s01 = block_sum(v01), s23 = block_sum(v23);
+    if (t == 0) {
+      cls_n[0] = s01 & 0xffff;
+      cls_n[1] = s01 >> 16;
+      cls_n[2] = s23 & 0xffff;
+      cls_n[3] = s23 >> 16;
     }
     __syncthreads();
   };
@@ -2020,14 +2029,13 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     bool fit;
     if (region > 0) {
       class_counts(L);
-      if (t == 0) {
-        int worst = 0;
-        for (int x = 0; x < 8; ++x) {
-          int v = 0;
-          for (int c = 0; c < 4; ++c) v += share(cls_n[c], x) * (8 >> c);
-          worst = max(worst, v);
-        }
-        fits = worst <= region;
+      if (t < 64) {  // lane x < 8: XCD x's waves; the wave's largest
+        int v = 0;
+        if (t < 8)
+          for (int c = 0; c < 4; ++c) v += share(cls_n[c], t) * (8 >> c);
+#pragma unroll
+        for (int o = 4; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
+        if (t == 0) fits = v <= region;
       }
       __syncthreads();
       fit = fits != 0;
@@ -2051,15 +2059,14 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
       carry[c] = region > 0 ? 0 : base;  // region: rank of the next super-tile of class c
       base += n * (8 >> c);
     }
-    if (region > 0)
-      for (int x = 0; x < 8; ++x) {
-        int o = x * region;
-        for (int c = 0; c < 4; ++c) {
-          xoff[x][c] = o;
-          o += share(cls_n[c], x) * (8 >> c);
-        }
-        xused[x] = o;
-      }
+  }
+  if (region > 0 && t < 8) {  // thread x: XCD x's first entry of each class
+    int o = t * region;
+    for (int c = 0; c < 4; ++c) {
+      xoff[t][c] = o;
+      o += share(cls_n[c], t) * (8 >> c);
+    }
+    xused[t] = o;
   }
   __syncthreads();
   __shared__ int wcnt[4][16];
