@@ -493,6 +493,12 @@ int fk_grid_build_cells(fk_grid *g, const float *xyz, const int64_t *grid_coord,
     return fail(FK_ERR_INVALID_ARG,
                 "fk_grid_build_cells: n in [0, 2^31), num_clouds >= 1, "
                 "num_down in [0, 30] and grid_coord required");
+  // Until this build succeeds the handle holds no usable build: a failure
+  // below may already have reallocated (or, on a failed hipMalloc, freed)
+  // tables that the previous build's sizes describe.
+  g->built = 0;
+  g->n_queries = -1;
+  ++g->generation;
   hipStream_t s = S(stream);
   const bool nosync = flags & FK_BUILD_NO_SYNC;
   if (nosync && (flags & FK_BUILD_COLUMNS))
@@ -525,6 +531,12 @@ int fk_grid_build_xyz(fk_grid *g, const float *xyz, const int32_t *offsets,
     return fail(FK_ERR_INVALID_ARG,
                 "fk_grid_build_xyz: n in [0, 2^31), num_clouds >= 1, xyz, "
                 "offsets and inv_cell required");
+  // Until this build succeeds the handle holds no usable build: a failure
+  // below may already have reallocated (or, on a failed hipMalloc, freed)
+  // tables that the previous build's sizes describe.
+  g->built = 0;
+  g->n_queries = -1;
+  ++g->generation;
   hipStream_t s = S(stream);
   const bool nosync = flags & FK_BUILD_NO_SYNC;
   if (nosync && (flags & FK_BUILD_COLUMNS))
@@ -565,6 +577,7 @@ int fk_grid_attach_queries(fk_grid *g, const float *qxyz, const int64_t *qcell,
   }
   hipStream_t s = S(stream);
   const bool nosync = flags & FK_BUILD_NO_SYNC;
+  g->n_queries = -1;  // set again once the query tables describe this list
   int64_t hint = g->n_cells;
   if (nosync || hint < 0) {
     // the query tables hold the build's table entries + 1 (one bucket for

@@ -211,6 +211,13 @@ int fk_grid_build_cells(fk_grid *grid, const float *xyz,
                         int64_t n, int32_t num_clouds, int32_t num_down,
                         int32_t flags, void *stream);
 
+/* A build that returns an error leaves the handle unbuilt: until a build
+ * succeeds, queries, attaches and table reads return FK_ERR_STATE (the
+ * failed build may already have reallocated tables that the earlier build
+ * described).  An attach that returns an error leaves no queries attached.
+ * (A no-sync build's device-side errors surface later, see
+ * FK_BUILD_NO_SYNC.) */
+
 /* Build from coordinates (exact-query layout): point i belongs to cloud c
  * with offsets[c-1] <= i < offsets[c] (int32 cumulative ends, device);
  * cell = floor(xyz * inv_cell[c]) (float[num_clouds], device).  Support ids

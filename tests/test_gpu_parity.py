@@ -559,6 +559,37 @@ def test_graph_replay_refused_after_rebuild():
     assert torch.equal(ri, ei) and torch.equal(rd, ed)
 
 
+def test_failed_build_leaves_the_handle_unbuilt(oracle):
+    """A build that returns an error leaves no usable build behind (it may
+    already have reallocated tables that the earlier build's sizes
+    describe): queries and attaches are refused with FK_ERR_STATE until a
+    build succeeds, and that build queries exactly."""
+    from flashknn_exprepo_amd import _lib as F
+    from flashknn_exprepo_amd.grid import Grid
+    rng = np.random.default_rng(73)
+    n = 3000
+    grid = rng.integers(0, 6, (n, 3))
+    g = Grid()
+    g.reserve(n, 1 << 16, 4, n)
+    g.build_cells(_t(grid), None, None, num_clouds=1, no_sync=True)
+    g.status()
+    g.query(8)
+    with pytest.raises(fk.FlashKNNError) as err:  # more points than reserved
+        g.build_cells(_t(rng.integers(0, 6, (2 * n, 3))), None, None, num_clouds=1,
+                      no_sync=True)
+    assert err.value.status == F.FK_ERR_CAPACITY
+    with pytest.raises(fk.FlashKNNError) as err:
+        g.query(8)
+    assert err.value.status == F.FK_ERR_STATE
+    with pytest.raises(fk.FlashKNNError) as err:
+        g.attach_queries(_t(grid.astype(np.float32)), qcell=_t(grid))
+    assert err.value.status == F.FK_ERR_STATE
+    g.build_cells(_t(grid), None, None, num_clouds=1)
+    idx, d = g.query(8)
+    ref_i, ref_d = oracle.flashknn_query(grid, None, None, 8, 0)
+    _same(idx, d, ref_i, ref_d)
+
+
 def test_batched_query_after_overflowing_no_sync_build():
     """A no-sync build whose boxes outgrow the reserve still records every
     cloud's base, past the cell table: a batched query on it reads no cell
