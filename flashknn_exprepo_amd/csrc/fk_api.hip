@@ -473,12 +473,24 @@ int fk_grid_reserve(fk_grid *g, int64_t max_points, int64_t max_cells,
   if (max_points < 0 || max_cells < 0 || max_clouds < 1 || max_queries < 0 ||
       max_points > INT32_MAX || max_cells > kHardCellCap)
     return fail(FK_ERR_INVALID_ARG, "fk_grid_reserve: bad sizes");
-  FK_CHECK(ensure_clouds(g, max_clouds));
-  FK_CHECK(ensure_points(g, max_points));
-  FK_CHECK(ensure_cells(g, max_cells));
-  FK_CHECK(ensure_qcells(g, max_cells));
+  // A table reallocated here loses its contents: a build whose tables moved
+  // is gone (queries are refused until the next build), and so are attached
+  // queries whose tables moved.  Checked on the failure paths too.
+  const uint64_t gen_build = g->generation;
+  int st = ensure_clouds(g, max_clouds);
+  if (st == FK_OK) st = ensure_points(g, max_points);
+  if (st == FK_OK) st = ensure_cells(g, max_cells);
+  if (g->generation != gen_build) {
+    g->built = 0;
+    g->n_queries = -1;
+  }
+  if (st != FK_OK) return st;
+  const uint64_t gen_queries = g->generation;
   const int64_t nq = max_queries > max_points ? max_queries : max_points;
-  FK_CHECK(ensure_queries(g, nq));
+  st = ensure_qcells(g, max_cells);
+  if (st == FK_OK) st = ensure_queries(g, nq);
+  if (g->generation != gen_queries) g->n_queries = -1;
+  if (st != FK_OK) return st;
   FK_CHECK(ensure_pending(g, nq, kMaxLaneK));
   g->reserved = 1;
   return ok();

@@ -197,7 +197,10 @@ int fk_morton_neighbor_keys(const int64_t *grid_down,
 /* ---------------------------------------------------------------- grids */
 int fk_grid_create(fk_grid **out);
 int fk_grid_destroy(fk_grid *grid);
-/* Pre-size the device tables (needed before FK_BUILD_NO_SYNC builds). */
+/* Pre-size the device tables (needed before FK_BUILD_NO_SYNC builds).  A
+ * reserve that grows the build's tables leaves the handle unbuilt (their
+ * contents are gone), one that grows the query tables leaves no queries
+ * attached: build (and attach) again before querying. */
 int fk_grid_reserve(fk_grid *grid, int64_t max_points, int64_t max_cells,
                     int32_t max_clouds, int64_t max_queries);
 

@@ -588,6 +588,15 @@ def test_failed_build_leaves_the_handle_unbuilt(oracle):
     idx, d = g.query(8)
     ref_i, ref_d = oracle.flashknn_query(grid, None, None, 8, 0)
     _same(idx, d, ref_i, ref_d)
+    # a reserve that reallocates the build's tables also leaves it unbuilt
+    g.reserve(4 * n, 1 << 18, 4, 4 * n)
+    with pytest.raises(fk.FlashKNNError) as err:
+        g.query(8)
+    assert err.value.status == F.FK_ERR_STATE
+    g.build_cells(_t(grid), None, None, num_clouds=1, no_sync=True)
+    g.status()
+    idx, d = g.query(8)
+    _same(idx, d, ref_i, ref_d)
 
 
 def test_batched_query_after_overflowing_no_sync_build():
