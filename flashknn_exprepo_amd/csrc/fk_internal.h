@@ -490,9 +490,13 @@ constexpr int kPlanTile = 64;
 // Plan entries per XCD when the plan is laid out per XCD (k_plan), else 0.
 // A plan of `waves` waves runs as waves / WPB blocks (WPB = 2 or 4), which
 // xcd_remap splits into eight equal ranges when waves is a multiple of 32
-// (always on a 256-CU part: waves = blocks per CU * CUs * WPB).
+// (always on a 256-CU part: waves = blocks per CU * CUs * WPB).  Off by
+// default (FK_PLAN_XCD=1 builds it): its L2 fetch is 22 % lower on the room
+// at K=16 with the same first-pass time, but the LiDAR sweep's first pass at
+// K=16 took 50.0 us instead of 45.8 and the room's at K=32 88.5 instead of
+// 85.8 with the waves dealt one by one over the XCDs.
 #ifndef FK_PLAN_XCD
-#define FK_PLAN_XCD 1
+#define FK_PLAN_XCD 0
 #endif
 __host__ __device__ constexpr int plan_xcd_region(int waves) {
   return FK_PLAN_XCD && waves > 0 && waves % 32 == 0 ? waves / 8 : 0;
