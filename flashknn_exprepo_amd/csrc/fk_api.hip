@@ -732,7 +732,7 @@ static int attach_plan(fk_grid *g, const fk_query_params *p, QueryArgs &a,
   int32_t cap = 0;
   FK_HIP(planned_capacity(kc, a.batch_sel, &cap));
   const int64_t ntiles = ((int64_t)a.nq + kPlanTile - 1) / kPlanTile;
-  if (!plan_fits(ntiles, cap) || ntiles > kPlanMaxTilesHost) return FK_OK;
+  if (ntiles > cap || ntiles > kPlanMaxTilesHost) return FK_OK;
   FK_CHECK(grow(g, g->plan, g->cap_plan, cap, "first-pass plan"));
   FK_CHECK(grow(g, g->tile_cost, g->cap_tcost, ntiles, "super-tile costs"));
   if (g->plan_generation != g->generation || g->plan_kc != kc || g->plan_R != a.R ||

@@ -487,37 +487,11 @@ hipError_t launch_query(const QueryArgs &a, hipStream_t s);
 // fewer.  Results do not depend on the plan.
 constexpr int kPlanTile = 64;
 
-// Plan entries per XCD when the plan is laid out per XCD (k_plan), else 0.
-// A plan of `waves` waves runs as waves / WPB blocks (WPB = 2 or 4), which
-// xcd_remap splits into eight equal ranges when waves is a multiple of 32
-// (always on a 256-CU part: waves = blocks per CU * CUs * WPB).  Off by
-// default (FK_PLAN_XCD=1 builds it): its L2 fetch is 22 % lower on the room
-// at K=16 with the same first-pass time, but the LiDAR sweep's first pass at
-// K=16 took 50.0 us instead of 45.8 and the room's at K=32 88.5 instead of
-// 85.8 with the waves dealt one by one over the XCDs.
-#ifndef FK_PLAN_XCD
-#define FK_PLAN_XCD 0
-#endif
-__host__ __device__ constexpr int plan_xcd_region(int waves) {
-  return FK_PLAN_XCD && waves > 0 && waves % 32 == 0 ? waves / 8 : 0;
-}
-// Runs per lane class in the per-XCD layout (a multiple of 8).  A class of
-// n super-tiles gives XCD x at most n / 8 + kPlanChunks / 8 of them, so the
-// layout needs super-tiles <= waves - kPlanChunks (plan_fits).
-#ifndef FK_PLAN_CHUNKS
-#define FK_PLAN_CHUNKS 64
-#endif
-constexpr int kPlanChunks = FK_PLAN_CHUNKS;
-static_assert(kPlanChunks % 8 == 0 && kPlanChunks >= 8, "plan runs: a multiple of 8");
-__host__ __device__ constexpr bool plan_fits(long long tiles, int waves) {
-  return plan_xcd_region(waves) > 0 ? tiles <= (long long)waves - kPlanChunks
-                                    : tiles <= (long long)waves;
-}
 // Resident waves of the planned first pass of K class kc and candidate
 // batch batch_sel (QueryArgs::batch_sel) on this device.
 hipError_t planned_capacity(int kc, int batch_sel, int32_t *waves);
 // Writes plan[0, capacity) (first query, lanes); tile_cost: int32[ceil(nq /
-// kPlanTile)] scratch.  Needs plan_fits(ceil(nq / kPlanTile), capacity).
+// kPlanTile)] scratch.  Needs nq <= kPlanTile * capacity.
 hipError_t launch_plan(const QueryArgs &a, int32_t capacity, int32_t *tile_cost,
                        int2 *plan, hipStream_t s);
 hipError_t launch_plan_columns(const QueryArgs &a, int32_t capacity,

@@ -1088,9 +1088,7 @@ k_query_planned(QueryArgs a) {
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
   if (blockIdx.x == 0 && threadIdx.x < kCntStride && a.qcnt_next) a.qcnt_next[threadIdx.x] = 0;
-  const int blk = plan_xcd_region(a.plan_waves) > 0 ? xcd_remap(blockIdx.x, gridDim.x)
-                                                    : (int)blockIdx.x;
-  const int wid = blk * WPB + w;
+  const int wid = blockIdx.x * WPB + w;
   if (wid >= a.plan_waves || a.scalars[kStatus] != 0) return;
   const int2 d = a.plan[wid];
   const int q0 = __builtin_amdgcn_readfirstlane(d.x);
@@ -1956,27 +1954,17 @@ __device__ __forceinline__ float plan_budget(int j) { return 4.f * exp2f(0.125f 
 // One block: the smallest budget whose waves fit `capacity`, then the
 // per-wave plan, super-tiles with more lanes first (they are the longest
 // waves: started first, they finish with the rest).  Unused plan entries
-// get (nq, 1) and exit at once.
-//
-// With region > 0 (capacity / 8; see plan_xcd_region) the plan is laid out
-// per XCD: entries [x * region, (x + 1) * region) are the waves that XCD x
-// runs (k_query_planned remaps its blocks with xcd_remap).  Each lane class
-// is cut into kPlanChunks runs of consecutive super-tiles and run j goes to
-// XCD j % 8, so every XCD gets the same mix of classes and kPlanChunks / 8
-// slabs of the cell-sorted queries spread over the cloud: its L2 fetches
-// the records of those slabs instead of most of the cloud (dealing single
-// waves round-robin over the XCDs spreads every slab over all eight), and
-// the slabs sample the cloud's dense and sparse parts alike (one slab per
-// XCD left some XCDs with more of the heavy waves).  The budget is the
-// smallest whose largest XCD share fits the region.
+// get (nq, 1) and exit at once.  The waves go to the XCDs one by one
+// (blocks are dealt round-robin over them): slabs of consecutive
+// super-tiles per XCD cut the L2 fetch by up to half but made the pass no
+// faster on the room and slower on a LiDAR sweep and at K = 32.
 #if FK_COMMON
 __global__ void __launch_bounds__(1024)
 k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
-       int32_t region, const int32_t *scalars, int2 *plan) {
+       const int32_t *scalars, int2 *plan) {
   __shared__ int cost[kPlanMaxTiles];
   __shared__ int red[16];
   __shared__ int cls_n[4], carry[4];
-  __shared__ int xoff[8][4], xused[8], fits;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const bool ok = scalars[kStatus] == 0;
   if (!ok) {
@@ -1995,10 +1983,22 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     __syncthreads();
     return s;
   };
-  // super-tiles per class (G = 8, 4, 2, 1) under budget L -> cls_n: the
-  // four counts in 16-bit fields of two sums (ntiles <= kPlanMaxTiles)
+  // waves(L) does not grow with L: bisect the budget levels
+  int lo = 0, hi = (1 << kPlanSearch) - 1;  // hi always fits (8192 per lane)
+  for (int it = 0; it < kPlanSearch; ++it) {
+    const int mid = (lo + hi) >> 1;
+    const float L = plan_budget(mid);
+    int v = 0;
+    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L);
+    if (block_sum(v) <= capacity) hi = mid; else lo = mid + 1;
+  }
+  // the last level puts every super-tile on kPlanGmin lanes per query
+  // (waves = kPlanGmin * super-tiles <= capacity, which the host checks)
+  const float L = hi == (1 << kPlanSearch) - 1 ? INFINITY : plan_budget(hi);
+  // super-tiles per class (G = 8, 4, 2, 1): the four counts in 16-bit
+  // fields of two sums (ntiles <= kPlanMaxTiles)
   static_assert(kPlanMaxTiles < (1 << 16), "16-bit class counts");
-  auto class_counts = [&](float L) {
+  {
     int v01 = 0, v23 = 0;
     for (int i = t; i < ntiles; i += 1024) {
       const int g = plan_lanes(cost[i], L);
@@ -2011,62 +2011,12 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
       cls_n[1] = s01 >> 16;
       cls_n[2] = s23 & 0xffff;
       cls_n[3] = s23 >> 16;
-    }
-    __syncthreads();
-  };
-  // class c's super-tiles in kPlanChunks runs; run j goes to XCD j % 8
-  auto csize = [](int n, int j) { return (j + 1) * n / kPlanChunks - j * n / kPlanChunks; };
-  auto share = [&](int n, int x) {  // XCD x's super-tiles of a class of n
-    int v = 0;
-    for (int j = x; j < kPlanChunks; j += 8) v += csize(n, j);
-    return v;
-  };
-  // waves(L) does not grow with L: bisect the budget levels
-  int lo = 0, hi = (1 << kPlanSearch) - 1;  // hi always fits (8192 per lane)
-  for (int it = 0; it < kPlanSearch; ++it) {
-    const int mid = (lo + hi) >> 1;
-    const float L = plan_budget(mid);
-    bool fit;
-    if (region > 0) {
-      class_counts(L);
-      if (t < 64) {  // lane x < 8: XCD x's waves; the wave's largest
-        int v = 0;
-        if (t < 8)
-          for (int c = 0; c < 4; ++c) v += share(cls_n[c], t) * (8 >> c);
-#pragma unroll
-        for (int o = 4; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));
-        if (t == 0) fits = v <= region;
+      int base = 0;
+      for (int c = 0; c < 4; ++c) {
+        carry[c] = base;  // running first wave of the next super-tile of class c
+        base += cls_n[c] * (8 >> c);
       }
-      __syncthreads();
-      fit = fits != 0;
-      __syncthreads();
-    } else {
-      int v = 0;
-      for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L);
-      fit = block_sum(v) <= capacity;
     }
-    if (fit) hi = mid; else lo = mid + 1;
-  }
-  // the last level puts every super-tile on kPlanGmin lanes per query
-  // (waves = kPlanGmin * super-tiles <= capacity, which the host checks;
-  // per XCD at most ceil(ntiles / 8) <= region, checked by plan_xcd_region)
-  const float L = hi == (1 << kPlanSearch) - 1 ? INFINITY : plan_budget(hi);
-  class_counts(L);
-  if (t == 0) {
-    int base = 0;
-    for (int c = 0; c < 4; ++c) {
-      const int n = cls_n[c];
-      carry[c] = region > 0 ? 0 : base;  // region: rank of the next super-tile of class c
-      base += n * (8 >> c);
-    }
-  }
-  if (region > 0 && t < 8) {  // thread x: XCD x's first entry of each class
-    int o = t * region;
-    for (int c = 0; c < 4; ++c) {
-      xoff[t][c] = o;
-      o += share(cls_n[c], t) * (8 >> c);
-    }
-    xused[t] = o;
   }
   __syncthreads();
   __shared__ int wcnt[4][16];
@@ -2084,16 +2034,7 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     if (g != 0) {
       int before = 0;  // super-tiles of class c ahead of this one in the chunk
       for (int j = 0; j < w; ++j) before += wcnt[c][j];
-      int first;
-      if (region > 0) {
-        const int n = cls_n[c], r = carry[c] + before + pos;  // rank in class c
-        const int j = (kPlanChunks * (r + 1) + n - 1) / n - 1;  // its run: start <= r
-        int o = r - j * n / kPlanChunks;  // ahead of it in XCD j % 8's share
-        for (int jj = j & 7; jj < j; jj += 8) o += csize(n, jj);
-        first = xoff[j & 7][c] + o * g;
-      } else {
-        first = carry[c] + (before + pos) * g;
-      }
+      const int first = carry[c] + (before + pos) * g;
       for (int p = 0; p < g; ++p)
         if (first + p < capacity)  // always (the waves fit by construction)
           plan[first + p] = make_int2(i * kPlanTile + p * (kPlanTile / g), g);
@@ -2102,14 +2043,9 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     if (t < 4) {
       int tot = 0;
       for (int j = 0; j < 16; ++j) tot += wcnt[t][j];
-      carry[t] += region > 0 ? tot : tot * (8 >> t);
+      carry[t] += tot * (8 >> t);
     }
     __syncthreads();
-  }
-  if (region > 0) {
-    for (int x = 0; x < 8; ++x)
-      for (int i = xused[x] + t; i < (x + 1) * region; i += 1024) plan[i] = make_int2(nq, 1);
-    return;
   }
   int used = 0;
   for (int c = 0; c < 4; ++c) used += cls_n[c] * (8 >> c);
@@ -2460,11 +2396,11 @@ hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
                                   int32_t *tile_cost, int2 *plan, hipStream_t s) {
   FK_FORWARD(launch_plan_columns(a, capacity, tile_cost, plan, s));
   const int64_t ntiles = ((int64_t)a.nq + kPlanTile - 1) / kPlanTile;
-  if (a.nq <= 0 || !plan_fits(ntiles * kPlanGmin, capacity) || ntiles > kPlanMaxTiles)
+  if (a.nq <= 0 || ntiles * kPlanGmin > capacity || ntiles > kPlanMaxTiles)
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_tile_cost, dim3((a.nq + 255) / 256), dim3(256), 0, s, a, tile_cost);
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, tile_cost, (int32_t)ntiles, a.nq,
-                     capacity, plan_xcd_region(capacity), a.scalars, plan);
+                     capacity, a.scalars, plan);
   return hipGetLastError();
 }
 

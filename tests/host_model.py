@@ -718,84 +718,3 @@ def run_case(engine, case, sel=None):
     off = np.cumsum([len(p) for p in case["parts"]]).astype(np.int64)
     engine(case["xyz"], off, case["kk"])
 
-
-# ------------------------------------------------ first-pass plan (k_plan)
-#
-# The per-XCD layout of the balanced plan (fk_query.hip k_plan, constants in
-# fk_internal.h): every super-tile's g waves must land in distinct entries of
-# its XCD's region, and the entries left over must be padding.
-
-PLAN_ONE_LANE_COST = 2.0
-PLAN_SEARCH = 7
-
-
-def plan_lanes(cost, L):
-    if cost * PLAN_ONE_LANE_COST <= L:
-        return 1
-    g = 2
-    while g < 8 and cost > g * L:
-        g <<= 1
-    return g
-
-
-def plan_budget(j):
-    return np.float32(4.0 * 2.0 ** (0.125 * j))
-
-
-def plan_fits(tiles, waves, chunks):
-    region = waves // 8 if waves > 0 and waves % 32 == 0 else 0
-    return tiles <= waves - chunks if region else tiles <= waves
-
-
-def plan_layout(cost, capacity, chunks):
-    """k_plan with region = capacity / 8: entries -> (tile, part) or None."""
-    region = capacity // 8
-    check(capacity % 32 == 0 and plan_fits(len(cost), capacity, chunks), "plan fits")
-
-    def csize(n, j):
-        return (j + 1) * n // chunks - j * n // chunks
-
-    def share(n, x):
-        return sum(csize(n, j) for j in range(x, chunks, 8))
-
-    def counts(L):
-        g = [plan_lanes(c, L) for c in cost]
-        return [sum(1 for v in g if v == (8 >> c)) for c in range(4)]
-
-    lo, hi = 0, (1 << PLAN_SEARCH) - 1
-    for _ in range(PLAN_SEARCH):
-        mid = (lo + hi) >> 1
-        n = counts(plan_budget(mid))
-        worst = max(sum(share(n[c], x) * (8 >> c) for c in range(4)) for x in range(8))
-        if worst <= region:
-            hi = mid
-        else:
-            lo = mid + 1
-    L = np.inf if hi == (1 << PLAN_SEARCH) - 1 else plan_budget(hi)
-    n = counts(L)
-    xoff = [[0] * 4 for _ in range(8)]
-    used = [0] * 8
-    for x in range(8):
-        o = x * region
-        for c in range(4):
-            xoff[x][c] = o
-            o += share(n[c], x) * (8 >> c)
-        used[x] = o
-        check(o <= (x + 1) * region, f"XCD {x} share within its region")
-    plan = [None] * capacity
-    rank = [0] * 4
-    for i, cst in enumerate(cost):
-        g = plan_lanes(cst, L)
-        c = {8: 0, 4: 1, 2: 2, 1: 3}[g]
-        r = rank[c]
-        rank[c] += 1
-        j = (chunks * (r + 1) + n[c] - 1) // n[c] - 1
-        check(j * n[c] // chunks <= r < (j + 1) * n[c] // chunks, "run of the super-tile")
-        o = r - j * n[c] // chunks + sum(csize(n[c], jj) for jj in range(j & 7, j, 8))
-        first = xoff[j & 7][c] + o * g
-        for p in range(g):
-            e = first + p
-            check((j & 7) * region <= e < used[j & 7], "entry inside its XCD's share")
-            check(plan[e] is None, "plan entry written once")
-            plan[e] = (i, p, g)
-    return plan, used, region

@@ -102,26 +102,3 @@ def test_attach_refuses_a_build_beyond_the_query_tables():
         h.attach(cell, cloud, hm.NO_SYNC)
     assert e.value.status == hm.ERR_CAPACITY
 
-
-@pytest.mark.parametrize("chunks", [8, 64])
-def test_plan_layout_per_xcd(chunks):
-    """Every super-tile of the per-XCD plan layout gets its g waves in
-    distinct entries of one XCD's region, all queries are covered once and
-    the rest of each region is padding (random cost profiles and sizes)."""
-    internal = (CSRC / "fk_internal.h").read_text()
-    assert re.search(r"#define FK_PLAN_CHUNKS 64", internal)
-    rng = np.random.default_rng(7)
-    for capacity in (3072, 2048, 1024):
-        for _ in range(6):
-            ntiles = int(rng.integers(1, capacity - chunks + 1))
-            cost = (rng.lognormal(5.0, 1.0, ntiles)).astype(np.int64)
-            plan, used, region = hm.plan_layout(cost, capacity, chunks)
-            covered = np.zeros(ntiles * 64, np.int32)
-            for e in plan:
-                if e is not None:
-                    i, p, g = e
-                    q0 = i * 64 + p * (64 // g)
-                    covered[q0:q0 + 64 // g] += 1
-            assert (covered == 1).all()
-            for x in range(8):
-                assert all(plan[e] is None for e in range(used[x], (x + 1) * region))
