@@ -2164,11 +2164,15 @@ __global__ void k_reset_query_counters(int32_t *qcnt) {
 #endif  // FK_COMMON
 
 // Fixed grid (graph-capturable): pending queries are read on the device.
-// Up to kExpandWaves waves, static striding over the pending list.  (One
-// wave per pending query, 16,384 waves, measured slower: LiDAR K=16 query
-// 179 -> 212 us.)
+// Up to kExpandWaves waves, static striding over the pending list: two
+// waves per SIMD, each finishing a few queries one after another, beat
+// more waves that share their SIMDs' issue (LiDAR sweep, 7,614 pending at
+// K=16, whole query: 1,024 / 2,048 / 3,072 / 4,096 / 8,192 waves 124 /
+// 108 / 112 / 121 / 159 us; the 16-cloud batched replay 0.52 / 0.468 /
+// 0.467 / 0.476 ms; one wave per pending query, 16,384 waves, slower
+// still).
 #ifndef FK_EXPAND_WAVES
-#define FK_EXPAND_WAVES 4096
+#define FK_EXPAND_WAVES 2048
 #endif
 constexpr int kExpandWaves = FK_EXPAND_WAVES;
 
