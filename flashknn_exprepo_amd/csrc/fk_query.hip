@@ -147,7 +147,6 @@ constexpr int lanes_per_query() {
   return KC == 8 ? FK_G_8 : KC == 16 ? FK_G_16 : KC == 32 ? FK_G_32 : FK_G_64;
 }
 
-
 template <int KC>
 constexpr int min_waves_per_eu() {
   return KC == 8 ? FK_WPE_8 : KC == 16 ? FK_WPE_16 : KC == 32 ? FK_WPE_32 : FK_WPE_64;
@@ -1727,10 +1726,11 @@ __host__ __device__ constexpr int batched_tile(int cls) {
 }
 static_assert(batched_class(64) == kBatchedClasses - 1, "batched classes");
 
-// Candidate batch of the batched first pass: the default of the K class,
-// the large one for K class 32 (its LiDAR and room clouds ran 8 % faster
-// with it alone, the 16-cloud batch 1 %).
-// FK_BATCHED_CB_<KC>: the BatchSel of each class's batched kernel.
+// Candidate batch of the batched first pass (FK_BATCHED_CB_<KC>: the
+// BatchSel of each class's kernel): the default of the K class, the large
+// one for K class 32 (its LiDAR and room clouds ran 8 % faster with it
+// alone, the 16-cloud batch 1-3 %; small batches for K classes 8/16 cost the
+// batch 13 %, large ones changed nothing).
 #ifndef FK_BATCHED_CB_8
 #define FK_BATCHED_CB_8 0
 #endif
@@ -1925,7 +1925,6 @@ __global__ void __launch_bounds__(256) k_tile_cost(const QueryArgs a, int32_t *t
 #endif  // FK_COMMON
 
 constexpr int kPlanMaxTiles = 8192;  // super-tiles cached in LDS by k_plan
-
 
 constexpr int kPlanSearch = 7;       // bisection steps over the 2^7 budgets
 

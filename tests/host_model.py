@@ -717,4 +717,3 @@ def run_case(engine, case, sel=None):
             pass
     off = np.cumsum([len(p) for p in case["parts"]]).astype(np.int64)
     engine(case["xyz"], off, case["kk"])
-

@@ -101,4 +101,3 @@ def test_attach_refuses_a_build_beyond_the_query_tables():
     with pytest.raises(hm.LibError) as e:
         h.attach(cell, cloud, hm.NO_SYNC)
     assert e.value.status == hm.ERR_CAPACITY
-
