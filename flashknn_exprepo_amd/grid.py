@@ -194,7 +194,11 @@ class Grid:
         return {"points": n.value, "clouds": c.value, "cells": cells.value, "queries": q.value}
 
     def tables(self):
-        """(sorted records float[N,4], cell_start int32[cells+1]) as tensors."""
+        """(sorted records float[N,4], cell_start int32[cells+1]) as tensors.
+
+        The tensors are views of the handle's own device tables: they are
+        valid until the next build or growing ``reserve`` on this grid
+        (which may reallocate the tables) and must not outlive the grid."""
         info = self.info()
         rec, cs = ctypes.c_void_p(), ctypes.c_void_p()
         F.check(self._L.fk_grid_tables(self._h, ctypes.byref(rec), ctypes.byref(cs)),
