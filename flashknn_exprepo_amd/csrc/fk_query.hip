@@ -1312,8 +1312,11 @@ struct CoopList {
 // and then walked in order.  (With one lane per column the wave waited for
 // its longest column: a room query's 15-column shell took 8 dependent
 // rounds of four loads, LiDAR shells through tall objects up to 100.)
+// Four columns per lane at a time: with two expansion waves per SIMD, 4
+// beat 6, 8, 12 and 16 (the smaller table also leaves the batched class
+// kernels more LDS beside the expansion).
 #ifndef FK_SHELL_BATCH
-#define FK_SHELL_BATCH 8
+#define FK_SHELL_BATCH 4
 #endif
 constexpr int kShellBatch = FK_SHELL_BATCH;  // column segments looked up together per lane
 constexpr int kShellTab = 2 * kShellBatch * kWave;  // segment table entries per wave
