@@ -1938,13 +1938,21 @@ constexpr int kPlanSearch = 7;       // bisection steps over the 2^7 budgets
 #ifndef FK_PLAN_ONE_LANE_COST
 #define FK_PLAN_ONE_LANE_COST 2.0f
 #endif
+// K class 32 (its lists fill a lane's registers either way, and two lanes
+// per query double the merge work) takes one lane at cost factor 1: room
+// 98k first pass 85.3 -> 81.7 us, LiDAR sweep 92k 110.8 -> 106.7 us; for K
+// classes 8/16 factors of 1 and 1.5 were slower (room K=16 43.5 -> 48.5 /
+// 44.3 us, LiDAR 46.4 -> 55.5 / 50.8 us).
+#ifndef FK_PLAN_ONE_LANE_COST_32
+#define FK_PLAN_ONE_LANE_COST_32 1.0f
+#endif
 constexpr int kPlanGmin = 1;
 
 // Lanes per query of a super-tile whose largest count is `cost` under the
 // per-lane budget L: 1 if cost * FK_PLAN_ONE_LANE_COST <= L, else the
 // smallest power of two in [2, 8] with cost <= G * L.
-__device__ __forceinline__ int plan_lanes(int cost, float L) {
-  if ((float)cost * FK_PLAN_ONE_LANE_COST <= L) return 1;
+__device__ __forceinline__ int plan_lanes(int cost, float L, float one_lane_cost) {
+  if ((float)cost * one_lane_cost <= L) return 1;
   int g = 2;
   while (g < 8 && (float)cost > (float)g * L) g <<= 1;
   return g;
@@ -1962,7 +1970,7 @@ __device__ __forceinline__ float plan_budget(int j) { return 4.f * exp2f(0.125f 
 // faster on the room and slower on a LiDAR sweep and at K = 32.
 #if FK_COMMON
 __global__ void __launch_bounds__(1024)
-k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
+k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity, float olc,
        const int32_t *scalars, int2 *plan) {
   __shared__ int cost[kPlanMaxTiles];
   __shared__ int red[16];
@@ -1991,7 +1999,7 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
     const int mid = (lo + hi) >> 1;
     const float L = plan_budget(mid);
     int v = 0;
-    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L);
+    for (int i = t; i < ntiles; i += 1024) v += plan_lanes(cost[i], L, olc);
     if (block_sum(v) <= capacity) hi = mid; else lo = mid + 1;
   }
   // the last level puts every super-tile on kPlanGmin lanes per query
@@ -2003,7 +2011,7 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
   {
     int v01 = 0, v23 = 0;
     for (int i = t; i < ntiles; i += 1024) {
-      const int g = plan_lanes(cost[i], L);
+      const int g = plan_lanes(cost[i], L, olc);
       v01 += g == 8 ? 1 : (g == 4 ? 1 << 16 : 0);
       v23 += g == 2 ? 1 : (g == 1 ? 1 << 16 : 0);
     }
@@ -2024,7 +2032,7 @@ k_plan(const int32_t *tile_cost, int32_t ntiles, int32_t nq, int32_t capacity,
   __shared__ int wcnt[4][16];
   for (int i0 = 0; i0 < ntiles; i0 += 1024) {
     const int i = i0 + t;
-    const int g = i < ntiles ? plan_lanes(cost[i], L) : 0;
+    const int g = i < ntiles ? plan_lanes(cost[i], L, olc) : 0;
     const int c = g == 8 ? 0 : (g == 4 ? 1 : (g == 2 ? 2 : 3));
     int pos = 0;
     for (int cc = 0; cc < 4; ++cc) {
@@ -2406,7 +2414,8 @@ hipError_t FK_LAYOUT(launch_plan)(const QueryArgs &a, int32_t capacity,
     return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_tile_cost, dim3((a.nq + 255) / 256), dim3(256), 0, s, a, tile_cost);
   hipLaunchKernelGGL(k_plan, dim3(1), dim3(1024), 0, s, tile_cost, (int32_t)ntiles, a.nq,
-                     capacity, a.scalars, plan);
+                     capacity, a.k > 16 ? FK_PLAN_ONE_LANE_COST_32 : FK_PLAN_ONE_LANE_COST,
+                     a.scalars, plan);
   return hipGetLastError();
 }
 
