@@ -104,7 +104,10 @@ namespace {
 #ifndef FK_WPE_64
 #define FK_WPE_64 1
 #endif
-//   FK_G_<KC>        lanes per query in the first pass (1, 2 or 4)
+//   FK_G_<KC>        lanes per query in the first pass (1, 2 or 4); K class
+//                    32 takes one (250k-point room, a launch of many more
+//                    tiles than wave slots: exact 201 -> 187 us, 27-cell
+//                    188 -> 171 us; K class 16 stays at two, 96 vs 121 us)
 //   FK_INLINE_PLAIN  the unplanned first pass (k_query) expands a tile's
 //                    lone uncertified query itself, as the planned one does
 #ifndef FK_INLINE_PLAIN
@@ -117,7 +120,7 @@ namespace {
 #define FK_G_16 2
 #endif
 #ifndef FK_G_32
-#define FK_G_32 2
+#define FK_G_32 1
 #endif
 #ifndef FK_G_64
 #define FK_G_64 1
@@ -2262,7 +2265,8 @@ void launch_plain(const QueryArgs &a, hipStream_t s) {
   if (G > 1 && a.one_lane) {
     with_batch<KC>(a.batch_sel,
                    [&](auto cb) { launch_first<KC, 1, decltype(cb)::value>(a, s); });
-  } else if constexpr (G > 1) {
+  } else if constexpr (G > 1 || KC <= 32) {
+    // the candidate batch of the hint; small launches widen their groups
     with_batch<KC>(a.batch_sel,
                    [&](auto cb) { launch_grouped<KC, G, decltype(cb)::value>(a, s); });
   } else {
